@@ -1,0 +1,156 @@
+/*
+ * mpm/mpm.h — C-ABI of the MI355X MLS-MPM snow solver (libmpm_hip.so).
+ *
+ * The reference exposes its solver as a compile-time, duck-typed class with a
+ * default constructor, void Step() and GetParticlePositions()
+ * (src/gpu/GPUSolver.h:10-17, src/cpu/CPUSolver.h:10-27, selected at
+ * src/main.cpp:22-29).  This header is the drop-in for that boundary, with the
+ * stages of GPUSolver::Step (src/gpu/GPUSolver.cpp:63-80) exported one by one.
+ *
+ * Conventions
+ *   - every entry point returns 0 on success and a negative MPM_E* code on error;
+ *     mpm_last_error() returns the message of the last error on the calling thread;
+ *     no C++ exception crosses this boundary;
+ *   - a solver handle owns all of its device memory and one HIP stream; it is not
+ *     thread-safe: callers serialise the calls on one handle;
+ *   - host arrays are FP32, structure-of-arrays, component-major:
+ *       x[d*n + p]            d < dim
+ *       C[(r*dim + c)*n + p]  row-major dim x dim  (C = D^-1 B, the MLS affine matrix)
+ *       F[(r*dim + c)*n + p]  elastic deformation gradient F_E
+ *       Jp[p]                 det of the plastic deformation gradient F_P
+ *     and always in the particle order given to mpm_create, whatever order the
+ *     solver keeps internally;
+ *   - grid arrays use the reference's node numbering k*(W+1)*(H+1) + j*(W+1) + i
+ *     (src/gpu/shaders/3d/p2g.glsl:75), component-major for vectors.
+ */
+#ifndef MPM_MPM_H
+#define MPM_MPM_H
+
+#include <stdint.h>
+#include "params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK            0
+#define MPM_E_INVALID    -1   /* bad argument or parameter                 */
+#define MPM_E_HIP        -2   /* HIP runtime error (allocation, launch)    */
+#define MPM_E_STATE      -3   /* call not valid in the solver's state      */
+#define MPM_E_NONFINITE  -4   /* NaN/Inf detected by mpm_check_finite      */
+
+typedef struct mpm_solver mpm_solver;
+
+/* Initial particle set (host memory).  NULL optional fields take the reference's
+ * initial values (GPUSolver.cpp:187-193): v = 0, C = 0, F = I, Jp = 1.
+ * mass/vol: per-particle arrays; the reference computes them per shape as
+ * vol = (shape area or volume)/count and mass = vol*RHO/100 (GPUSolver.cpp:117-119,147-148). */
+typedef struct mpm_particles {
+    int64_t      n;
+    const float* x;
+    const float* v;
+    const float* mass;
+    const float* vol;
+    const float* C;
+    const float* F;
+    const float* Jp;
+} mpm_particles;
+
+/* Solver options beyond the physics parameters (the reference has none). */
+typedef struct mpm_options {
+    int sort_every;     /* re-bin and reorder particles every k substeps of mpm_step; 0 = never */
+    int fused;          /* 1: mpm_step runs the fused G2P+P2G pipeline; 0: the four reference stages */
+    int tile_particles; /* particles per work item of the tiled kernels (0 = default)            */
+} mpm_options;
+
+/* Parameters.h defaults (src/shared/Parameters.h:30-84) for dim 2 or 3. */
+int mpm_params_reference(int dim, mpm_params* out);
+/* Recompute MU and LAMBDA from E and NU as Parameters.h:82-84 does. */
+int mpm_params_derive_lame(mpm_params* p);
+/* Default options. */
+int mpm_options_default(mpm_options* out);
+
+/* GPUSolver::GPUSolver (GPUSolver.cpp:7-51): allocate and upload; opts may be NULL. */
+int mpm_create(const mpm_params* params, const mpm_particles* init,
+               const mpm_options* opts, mpm_solver** out);
+/* GPUSolver::~GPUSolver (GPUSolver.cpp:53-60). */
+int mpm_destroy(mpm_solver* s);
+
+/* GPUSolver::Step (GPUSolver.cpp:63-80), n_substeps times, asynchronous on the
+ * solver's stream. */
+int mpm_step(mpm_solver* s, int n_substeps);
+
+/* Stage-level drop-ins, one per dispatch of GPUSolver::Step:
+ *   mpm_p2g          shaders/{2d,3d}/p2g.glsl           (adds into the grid)
+ *   mpm_grid_update  shaders/{2d,3d}/grid_update.glsl
+ *   mpm_g2p          shaders/{2d,3d}/g2p.glsl
+ *   mpm_reset_grid   shaders/{2d,3d}/reset.glsl
+ * Sequencing them p2g, grid_update, g2p, reset is one reference substep. */
+int mpm_p2g(mpm_solver* s);
+int mpm_grid_update(mpm_solver* s);
+int mpm_g2p(mpm_solver* s);
+int mpm_reset_grid(mpm_solver* s);
+
+/* Particle binning: key = cell id in block-major order, stable LSD radix sort,
+ * reorder of every particle array.  No reference counterpart (the reference
+ * scatters in particle order, shaders/2d/p2g.glsl:88-90). */
+int mpm_sort(mpm_solver* s);
+/* After mpm_sort: the sorted keys and, for each sorted slot, the original
+ * particle index (n entries each; either pointer may be NULL). */
+int mpm_get_sort(mpm_solver* s, uint32_t* keys, uint32_t* perm);
+
+/* Wait for all work on the solver's stream (the reference's glFinish,
+ * GPUSolver.cpp:278). */
+int mpm_sync(mpm_solver* s);
+
+/* GPUSolver::GetParticlePositions (GPUSolver.cpp:276-295): synchronises and copies
+ * positions, interleaved [p*dim + d] as std::vector<Vec> holds them. */
+int mpm_get_positions(mpm_solver* s, float* xyz);
+
+/* Full state readback / upload (SoA layout above; NULL pointers are skipped). */
+int mpm_get_state(mpm_solver* s, float* x, float* v, float* C, float* F, float* Jp);
+int mpm_set_state(mpm_solver* s, const float* x, const float* v, const float* C,
+                  const float* F, const float* Jp);
+
+/* Grid readback in the reference's node numbering: after mpm_p2g mass and
+ * momentum; after mpm_grid_update the collision and friction velocities
+ * (gVelCol / gVelFric).  Any pointer may be NULL. */
+int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfric);
+
+int mpm_num_particles(const mpm_solver* s, int64_t* n);
+int mpm_num_nodes(const mpm_params* p, int64_t* nodes);
+
+/* Device-side timing of the last mpm_step call (HIP events around it on the
+ * solver's stream), milliseconds; and per-kernel average durations of the kernels
+ * mpm_step launched since mpm_reset_timers (kernel ids: MPM_KERNEL_*). */
+enum {
+    MPM_KERNEL_P2G = 0,
+    MPM_KERNEL_GRID_UPDATE = 1,
+    MPM_KERNEL_G2P = 2,
+    MPM_KERNEL_RESET = 3,
+    MPM_KERNEL_G2P2G = 4,
+    MPM_KERNEL_SORT = 5,
+    MPM_KERNEL_COUNT = 6
+};
+int mpm_set_timing(mpm_solver* s, int enabled);
+int mpm_reset_timers(mpm_solver* s);
+int mpm_get_kernel_time(mpm_solver* s, int kernel, double* total_ms, int64_t* launches);
+
+/* Scan the particle state for NaN/Inf (the reference's NaN trap,
+ * src/cpu/Particles.cpp:50-61, as a status instead of a spin): returns
+ * MPM_E_NONFINITE and the count of bad particles. */
+int mpm_check_finite(mpm_solver* s, int64_t* bad_particles);
+
+/* Stable sort of n 32-bit keys on the device (the binning sort on its own, for
+ * callers with their own keys).  Host arrays; out_perm[i] = index of the i-th
+ * smallest key, ties in input order. key_bits <= 32. */
+int mpm_sort_pairs_host(const uint32_t* keys, int64_t n, int key_bits,
+                        uint32_t* out_keys, uint32_t* out_perm);
+
+const char* mpm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_MPM_H */

@@ -1,0 +1,804 @@
+// mpm_capi.hip — the C-ABI of libmpm_hip.so (include/mpm/mpm.h).
+//
+// Host side of the reference's GPUSolver (src/gpu/GPUSolver.{h,cpp}): owns the
+// device buffers and one HIP stream, uploads the initial particle set, sequences
+// the substep kernels and reads state back.  No exception crosses the ABI; every
+// entry point returns an MPM_* status and records a message for mpm_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm/mpm.h"
+#include "mpm_kernels.hpp"
+#include "mpm_sort.hpp"
+
+using namespace mpm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(MPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+enum GridState { GRID_CLEAN = 0, GRID_READY = 1, GRID_DIRTY = 2 };
+
+// One set of particle arrays on the device (the solver keeps two for reordering).
+struct ParticleBuffers {
+    void* base = nullptr;
+    float *x = nullptr, *v = nullptr, *C = nullptr, *F = nullptr, *Jp = nullptr;
+    float *mass = nullptr, *vol = nullptr;
+    uint32_t* pid = nullptr;
+};
+
+struct KernelTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+}  // namespace
+
+struct mpm_solver {
+    mpm_params params{};
+    mpm_options opts{};
+    Phys ph{};
+    LaunchCfg cfg{};
+    int dim = 2;
+    int n = 0;
+    hipStream_t stream = nullptr;
+
+    ParticleBuffers pb[2];
+    int cur_pb = 0;
+
+    // grids: a[2], vc[2], flags[2]
+    int nbx = 0, nby = 0, nbz = 0, nblocks = 0;
+    float4* ga[2] = {nullptr, nullptr};
+    float4* gvc[2] = {nullptr, nullptr};
+    uint32_t* gflags[2] = {nullptr, nullptr};
+    int cur_grid = 0;
+    int grid_state = GRID_CLEAN;
+
+    // binning
+    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr;
+    uint32_t* radix_scratch = nullptr;
+    int key_bits = 0;
+    int4* items = nullptr;
+    int* n_items = nullptr;
+    int max_items = 0;
+    bool items_valid = false;
+    bool sorted_keys_in_alt = false;
+    int since_sort = 0;
+
+    unsigned long long* d_bad = nullptr;
+
+    bool timing = false;
+    KernelTimer timers[MPM_KERNEL_COUNT];
+    std::vector<hipEvent_t> event_pool;
+
+    DevParticles dev_particles() const
+    {
+        const ParticleBuffers& b = pb[cur_pb];
+        DevParticles p;
+        p.x = b.x; p.v = b.v; p.C = b.C; p.F = b.F; p.Jp = b.Jp;
+        p.mass = b.mass; p.vol = b.vol; p.pid = b.pid; p.n = n;
+        return p;
+    }
+    DevGrid dev_grid(int which) const
+    {
+        DevGrid g;
+        g.a = ga[which];
+        g.vc = gvc[which];
+        g.flags = gflags[which];
+        g.nbx = nbx; g.nby = nby; g.nbz = nbz;
+        return g;
+    }
+    hipEvent_t take_event()
+    {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+namespace {
+
+// Wraps one kernel launch with HIP events when timing is on.
+template <typename Fn>
+int timed(mpm_solver* s, int kernel, Fn&& fn)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (s->timing) {
+        a = s->take_event();
+        b = s->take_event();
+        if (a) HIP_TRY(hipEventRecord(a, s->stream));
+    }
+    hipError_t e = fn();
+    if (e != hipSuccess) return fail(MPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (s->timing && a && b) {
+        HIP_TRY(hipEventRecord(b, s->stream));
+        s->timers[kernel].pending.emplace_back(a, b);
+    }
+    return MPM_OK;
+}
+
+int collect_timers(mpm_solver* s)
+{
+    for (int k = 0; k < MPM_KERNEL_COUNT; ++k) {
+        KernelTimer& t = s->timers[k];
+        for (auto& pr : t.pending) {
+            HIP_TRY(hipEventSynchronize(pr.second));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+            t.total_ms += ms;
+            t.launches += 1;
+            s->event_pool.push_back(pr.first);
+            s->event_pool.push_back(pr.second);
+        }
+        t.pending.clear();
+    }
+    return MPM_OK;
+}
+
+Phys make_phys(const mpm_params& p)
+{
+    Phys ph{};
+    ph.dx = p.dx;
+    ph.inv_dx = 1.0f / p.dx;
+    ph.dt = p.DELTA_TIME;
+    ph.Dinv = (p.spline_order == 3 ? 3.0f : 4.0f) * ph.inv_dx * ph.inv_dx;
+    ph.gravity = p.GRAVITY;
+    ph.friction = p.FRICTION;
+    ph.margin = p.BORDER_MARGIN;
+    ph.theta0 = 1.0f - p.THETA_C;
+    ph.theta1 = 1.0f + p.THETA_S;
+    ph.xi = p.XI;
+    ph.mu = p.MU;
+    ph.lambda = p.LAMBDA;
+    ph.mass_eps = p.mass_eps;
+    ph.W = p.WIDTH;
+    ph.H = p.HEIGHT;
+    ph.D = p.dim == 3 ? p.DEPTH : 0;
+    ph.boundary = p.boundary;
+    return ph;
+}
+
+int validate_params(const mpm_params* p)
+{
+    if (!p) return fail(MPM_E_INVALID, "params is NULL");
+    if (p->dim != 2 && p->dim != 3) return fail(MPM_E_INVALID, "dim must be 2 or 3");
+    if (p->spline_order != 2 && p->spline_order != 3)
+        return fail(MPM_E_INVALID, "spline_order must be 2 (quadratic) or 3 (cubic)");
+    if (p->boundary != MPM_BOUNDARY_COULOMB_MARGIN && p->boundary != MPM_BOUNDARY_STICKY)
+        return fail(MPM_E_INVALID, "unknown boundary kind");
+    if (p->WIDTH < 1 || p->HEIGHT < 1 || (p->dim == 3 && p->DEPTH < 1))
+        return fail(MPM_E_INVALID, "grid size must be positive");
+    if (!(p->dx > 0.f) || !(p->DELTA_TIME > 0.f)) return fail(MPM_E_INVALID, "dx and DELTA_TIME must be > 0");
+    int64_t nodes = 0;
+    mpm_num_nodes(p, &nodes);
+    // block-major node addresses and sort keys are 32-bit
+    int bs = p->dim == 3 ? 4 : 8;
+    int64_t nb = (int64_t)((p->WIDTH + bs) / bs) * ((p->HEIGHT + bs) / bs) * (p->dim == 3 ? (p->DEPTH + bs) / bs : 1);
+    if (nb * 64 >= ((int64_t)1 << 31)) return fail(MPM_E_INVALID, "grid too large for 32-bit node addressing");
+    return MPM_OK;
+}
+
+int alloc_particles(mpm_solver* s, ParticleBuffers& b)
+{
+    const int d = s->dim;
+    const size_t n = (size_t)std::max(s->n, 1);
+    size_t nf = (size_t)(d + d + d * d + d * d + 3) * n;
+    size_t bytes = nf * sizeof(float) + n * sizeof(uint32_t) + 256;
+    HIP_TRY(hipMalloc(&b.base, bytes));
+    float* f = static_cast<float*>(b.base);
+    b.x = f; f += d * n;
+    b.v = f; f += d * n;
+    b.C = f; f += d * d * n;
+    b.F = f; f += d * d * n;
+    b.Jp = f; f += n;
+    b.mass = f; f += n;
+    b.vol = f; f += n;
+    b.pid = reinterpret_cast<uint32_t*>(f);
+    return MPM_OK;
+}
+
+int reset_grid(mpm_solver* s, int which)
+{
+    return timed(s, MPM_KERNEL_RESET, [&] { return launch_reset(s->cfg, s->dev_grid(which), s->stream); });
+}
+
+int reset_all_grids(mpm_solver* s)
+{
+    int r = reset_grid(s, 0);
+    if (r) return r;
+    r = reset_grid(s, 1);
+    if (r) return r;
+    s->grid_state = GRID_CLEAN;
+    return MPM_OK;
+}
+
+int do_sort(mpm_solver* s)
+{
+    const int n = s->n;
+    auto body = [&]() -> hipError_t {
+        DevParticles p = s->dev_particles();
+        hipError_t e = launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
+        if (e != hipSuccess) return e;
+        bool in_alt = false;
+        e = radix_sort_pairs(s->keys, s->vals, s->keys_alt, s->vals_alt, n, s->key_bits, true,
+                             s->radix_scratch, s->stream, &in_alt);
+        if (e != hipSuccess) return e;
+        s->sorted_keys_in_alt = in_alt;
+        const uint32_t* perm = in_alt ? s->vals_alt : s->vals;
+        const uint32_t* skeys = in_alt ? s->keys_alt : s->keys;
+        ParticleBuffers& src = s->pb[s->cur_pb];
+        ParticleBuffers& dst = s->pb[1 - s->cur_pb];
+        const int d = s->dim;
+        struct { const float* a; float* b; int c; } fields[] = {
+            {src.x, dst.x, d}, {src.v, dst.v, d}, {src.C, dst.C, d * d}, {src.F, dst.F, d * d},
+            {src.Jp, dst.Jp, 1}, {src.mass, dst.mass, 1}, {src.vol, dst.vol, 1}};
+        for (auto& f : fields) {
+            e = launch_gather_fields(f.a, f.b, perm, n, f.c, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        e = launch_gather_u32(src.pid, dst.pid, perm, n, s->stream);
+        if (e != hipSuccess) return e;
+        s->cur_pb = 1 - s->cur_pb;
+        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->n_items, s->max_items,
+                                 s->stream);
+    };
+    int r = timed(s, MPM_KERNEL_SORT, body);
+    if (r) return r;
+    s->items_valid = true;
+    s->since_sort = 0;
+    return MPM_OK;
+}
+
+TileWork tile_work(const mpm_solver* s)
+{
+    TileWork w;
+    w.items = s->items;
+    w.n_items = s->n_items;
+    w.max_items = s->max_items;
+    return w;
+}
+
+int step_fused(mpm_solver* s, int nsub)
+{
+    int r;
+    if (!s->items_valid && (r = do_sort(s))) return r;
+    if (s->grid_state == GRID_DIRTY && (r = reset_all_grids(s))) return r;
+    if (s->grid_state != GRID_READY) {
+        DevParticles p = s->dev_particles();
+        DevGrid g = s->dev_grid(s->cur_grid);
+        TileWork w = tile_work(s);
+        r = timed(s, MPM_KERNEL_P2G, [&] { return launch_p2g_tiled(s->cfg, p, g, w, s->ph, s->stream); });
+        if (r) return r;
+        s->grid_state = GRID_READY;
+    }
+    for (int k = 0; k < nsub; ++k) {
+        if (s->opts.sort_every > 0 && s->since_sort >= s->opts.sort_every && (r = do_sort(s))) return r;
+        DevGrid cur = s->dev_grid(s->cur_grid), old = s->dev_grid(1 - s->cur_grid);
+        r = timed(s, MPM_KERNEL_GRID_UPDATE,
+                  [&] { return launch_grid_update_clear(s->cfg, cur, old, s->ph, s->stream); });
+        if (r) return r;
+        DevParticles p = s->dev_particles();
+        TileWork w = tile_work(s);
+        r = timed(s, MPM_KERNEL_G2P2G,
+                  [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->stream); });
+        if (r) return r;
+        s->cur_grid = 1 - s->cur_grid;
+        s->since_sort += 1;
+    }
+    return MPM_OK;
+}
+
+int step_staged(mpm_solver* s, int nsub)
+{
+    int r;
+    if (s->grid_state != GRID_CLEAN && (r = reset_all_grids(s))) return r;
+    for (int k = 0; k < nsub; ++k) {
+        if (s->opts.sort_every > 0 && s->since_sort >= s->opts.sort_every && (r = do_sort(s))) return r;
+        DevParticles p = s->dev_particles();
+        DevGrid g = s->dev_grid(s->cur_grid);
+        if ((r = timed(s, MPM_KERNEL_P2G, [&] { return launch_p2g(s->cfg, p, g, s->ph, s->stream); }))) return r;
+        if ((r = timed(s, MPM_KERNEL_GRID_UPDATE,
+                       [&] { return launch_grid_update(s->cfg, g, s->ph, s->stream); }))) return r;
+        if ((r = timed(s, MPM_KERNEL_G2P, [&] { return launch_g2p(s->cfg, p, g, s->ph, s->stream); }))) return r;
+        if ((r = reset_grid(s, s->cur_grid))) return r;
+        s->since_sort += 1;
+    }
+    s->grid_state = GRID_CLEAN;
+    return MPM_OK;
+}
+
+// Stage calls see the reference's state between Steps: a zero grid.
+int prepare_stage_call(mpm_solver* s)
+{
+    if (s->grid_state == GRID_READY) return reset_all_grids(s);
+    return MPM_OK;
+}
+
+int copy_to_host(mpm_solver* s, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return MPM_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_params_derive_lame(mpm_params* p)
+{
+    if (!p) return fail(MPM_E_INVALID, "params is NULL");
+    // Parameters.h:82-84, in float as there
+    p->MU = p->E / (2.f * (1.f + p->NU));
+    p->LAMBDA = (p->E * p->NU) / ((1.f + p->NU) * (1.f - (2.f * p->NU)));
+    return MPM_OK;
+}
+
+int mpm_params_reference(int dim, mpm_params* out)
+{
+    if (!out) return fail(MPM_E_INVALID, "out is NULL");
+    if (dim != 2 && dim != 3) return fail(MPM_E_INVALID, "dim must be 2 or 3");
+    mpm_params p{};
+    p.dim = dim;
+    p.WIDTH = 200;           // Parameters.h:31-33
+    p.HEIGHT = 100;
+    p.DEPTH = 100;           // Parameters.h:36
+    p.BORDER_MARGIN = 2.f;
+    p.spline_order = 3;      // cubic B-spline, shaders/2d/common.glsl:43-69
+    p.boundary = MPM_BOUNDARY_COULOMB_MARGIN;
+    p.dx = 1.f;              // grid units
+    p.GRAVITY = -9.81f;      // Parameters.h:58-60
+    p.FRICTION = 0.3f;
+    p.DELTA_TIME = 1e-3f;    // Parameters.h:68
+    p.THETA_C = 2.0e-2f;     // Parameters.h:74-79
+    p.THETA_S = 6.0e-3f;
+    p.XI = 10.f;
+    p.NU = 2.0e-1f;
+    p.E = 1.4e5f;
+    p.RHO = 400.f;
+    p.mass_eps = 1e-6f;      // shaders/2d/grid_update.glsl:90
+    mpm_params_derive_lame(&p);
+    *out = p;
+    return MPM_OK;
+}
+
+int mpm_options_default(mpm_options* out)
+{
+    if (!out) return fail(MPM_E_INVALID, "out is NULL");
+    out->sort_every = 16;
+    out->fused = 1;
+    out->tile_particles = 1024;
+    return MPM_OK;
+}
+
+int mpm_num_nodes(const mpm_params* p, int64_t* nodes)
+{
+    if (!p || !nodes) return fail(MPM_E_INVALID, "NULL argument");
+    int64_t g = (int64_t)(p->WIDTH + 1) * (p->HEIGHT + 1);
+    if (p->dim == 3) g *= (p->DEPTH + 1);
+    *nodes = g;
+    return MPM_OK;
+}
+
+int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_options* opts,
+               mpm_solver** out)
+{
+    if (!out) return fail(MPM_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int r = validate_params(params);
+    if (r) return r;
+    if (!init) return fail(MPM_E_INVALID, "init is NULL");
+    if (init->n < 0 || init->n > (int64_t)INT32_MAX / 16) return fail(MPM_E_INVALID, "particle count out of range");
+    if (init->n > 0 && (!init->x || !init->mass || !init->vol))
+        return fail(MPM_E_INVALID, "x, mass and vol are required");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
+        return fail(MPM_E_HIP, "no HIP device available");
+
+    mpm_solver* s = new mpm_solver();
+    s->params = *params;
+    if (opts) s->opts = *opts; else mpm_options_default(&s->opts);
+    if (s->opts.tile_particles <= 0) s->opts.tile_particles = 1024;
+    if (s->opts.sort_every < 0) s->opts.sort_every = 0;
+    s->dim = params->dim;
+    s->n = (int)init->n;
+    s->ph = make_phys(*params);
+    s->cfg.dim = params->dim;
+    s->cfg.order = params->spline_order;
+    s->cfg.boundary = params->boundary;
+
+    auto cleanup_fail = [&](int code) {
+        mpm_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup_fail(fail(MPM_E_HIP, "hipStreamCreate failed"));
+
+    const int bs = s->dim == 3 ? 4 : 8;
+    s->nbx = (params->WIDTH + 1 + bs - 1) / bs;
+    s->nby = (params->HEIGHT + 1 + bs - 1) / bs;
+    s->nbz = s->dim == 3 ? (params->DEPTH + 1 + bs - 1) / bs : 1;
+    s->nblocks = s->nbx * s->nby * s->nbz;
+    const size_t gbytes = (size_t)s->nblocks * 64 * sizeof(float4);
+    const bool coul = params->boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+    for (int g = 0; g < 2; ++g) {
+        if (hipMalloc(&s->ga[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
+        if (hipMemsetAsync(s->ga[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
+        if (coul) {
+            if (hipMalloc(&s->gvc[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
+            if (hipMemsetAsync(s->gvc[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
+        }
+        if (hipMalloc(&s->gflags[g], (size_t)s->nblocks * sizeof(uint32_t)) != hipSuccess)
+            return cleanup_fail(fail(MPM_E_HIP, "flag allocation failed"));
+        if (hipMemsetAsync(s->gflags[g], 0, (size_t)s->nblocks * sizeof(uint32_t), s->stream) != hipSuccess)
+            return cleanup_fail(fail(MPM_E_HIP, "memset"));
+    }
+
+    for (int b = 0; b < 2; ++b)
+        if ((r = alloc_particles(s, s->pb[b]))) return cleanup_fail(r);
+
+    // binning buffers
+    const size_t nn = (size_t)std::max(s->n, 1);
+    uint64_t maxkey = (uint64_t)s->nblocks * 64;
+    s->key_bits = 1;
+    while (s->key_bits < 32 && ((uint64_t)1 << s->key_bits) < maxkey) ++s->key_bits;
+    if (hipMalloc(&s->keys, nn * 4) != hipSuccess || hipMalloc(&s->vals, nn * 4) != hipSuccess ||
+        hipMalloc(&s->keys_alt, nn * 4) != hipSuccess || hipMalloc(&s->vals_alt, nn * 4) != hipSuccess ||
+        hipMalloc(&s->radix_scratch, radix_scratch_words(s->n) * 4) != hipSuccess)
+        return cleanup_fail(fail(MPM_E_HIP, "binning allocation failed"));
+    s->max_items = std::min(s->n, s->nblocks) + s->n / s->opts.tile_particles + 1;
+    if (hipMalloc(&s->items, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&s->n_items, sizeof(int)) != hipSuccess || hipMalloc(&s->d_bad, sizeof(unsigned long long)) != hipSuccess)
+        return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
+    if (hipMemsetAsync(s->n_items, 0, sizeof(int), s->stream) != hipSuccess)
+        return cleanup_fail(fail(MPM_E_HIP, "memset"));
+
+    // initial state (GPUSolver.cpp:100-159 defaults for omitted fields)
+    const int d = s->dim, n = s->n;
+    if (n > 0) {
+        std::vector<float> host((size_t)(d + d + d * d + d * d + 3) * n);
+        float* x = host.data();
+        float* v = x + (size_t)d * n;
+        float* C = v + (size_t)d * n;
+        float* F = C + (size_t)d * d * n;
+        float* Jp = F + (size_t)d * d * n;
+        float* m = Jp + n;
+        float* vol = m + n;
+        std::memcpy(x, init->x, sizeof(float) * d * n);
+        if (init->v) std::memcpy(v, init->v, sizeof(float) * d * n);
+        if (init->C) std::memcpy(C, init->C, sizeof(float) * d * d * n);
+        if (init->F) std::memcpy(F, init->F, sizeof(float) * d * d * n);
+        else
+            for (int e = 0; e < d * d; ++e)
+                std::fill(F + (size_t)e * n, F + (size_t)(e + 1) * n, (e % (d + 1)) == 0 ? 1.f : 0.f);
+        if (init->Jp) std::memcpy(Jp, init->Jp, sizeof(float) * n);
+        else std::fill(Jp, Jp + n, 1.f);
+        std::memcpy(m, init->mass, sizeof(float) * n);
+        std::memcpy(vol, init->vol, sizeof(float) * n);
+        std::vector<uint32_t> pid(n);
+        for (int i = 0; i < n; ++i) pid[i] = (uint32_t)i;
+        ParticleBuffers& b = s->pb[0];
+        if (hipMemcpyAsync(b.x, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(b.pid, pid.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            return cleanup_fail(fail(MPM_E_HIP, "upload failed"));
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "upload sync failed"));
+    } else if (hipStreamSynchronize(s->stream) != hipSuccess) {
+        return cleanup_fail(fail(MPM_E_HIP, "sync failed"));
+    }
+    *out = s;
+    return MPM_OK;
+}
+
+int mpm_destroy(mpm_solver* s)
+{
+    if (!s) return MPM_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& t : s->timers)
+        for (auto& pr : t.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    for (int g = 0; g < 2; ++g) {
+        (void)hipFree(s->ga[g]);
+        (void)hipFree(s->gvc[g]);
+        (void)hipFree(s->gflags[g]);
+        (void)hipFree(s->pb[g].base);
+    }
+    (void)hipFree(s->keys); (void)hipFree(s->vals); (void)hipFree(s->keys_alt); (void)hipFree(s->vals_alt);
+    (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return MPM_OK;
+}
+
+int mpm_step(mpm_solver* s, int n_substeps)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (n_substeps < 0) return fail(MPM_E_INVALID, "n_substeps < 0");
+    if (s->n == 0 || n_substeps == 0) return MPM_OK;
+    return s->opts.fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
+}
+
+int mpm_p2g(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    int r = prepare_stage_call(s);
+    if (r) return r;
+    DevParticles p = s->dev_particles();
+    DevGrid g = s->dev_grid(s->cur_grid);
+    r = timed(s, MPM_KERNEL_P2G, [&] { return launch_p2g(s->cfg, p, g, s->ph, s->stream); });
+    s->grid_state = GRID_DIRTY;
+    return r;
+}
+
+int mpm_grid_update(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    int r = prepare_stage_call(s);
+    if (r) return r;
+    DevGrid g = s->dev_grid(s->cur_grid);
+    r = timed(s, MPM_KERNEL_GRID_UPDATE, [&] { return launch_grid_update(s->cfg, g, s->ph, s->stream); });
+    s->grid_state = GRID_DIRTY;
+    return r;
+}
+
+int mpm_g2p(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    int r = prepare_stage_call(s);
+    if (r) return r;
+    DevParticles p = s->dev_particles();
+    DevGrid g = s->dev_grid(s->cur_grid);
+    r = timed(s, MPM_KERNEL_G2P, [&] { return launch_g2p(s->cfg, p, g, s->ph, s->stream); });
+    s->grid_state = GRID_DIRTY;
+    return r;
+}
+
+int mpm_reset_grid(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    return reset_all_grids(s);
+}
+
+int mpm_sort(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    return do_sort(s);
+}
+
+int mpm_get_sort(mpm_solver* s, uint32_t* keys, uint32_t* perm)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (!s->items_valid) return fail(MPM_E_STATE, "no sort has run");
+    const uint32_t* sk = s->sorted_keys_in_alt ? s->keys_alt : s->keys;
+    int r;
+    if (keys && (r = copy_to_host(s, keys, sk, (size_t)s->n * 4))) return r;
+    if (perm && (r = copy_to_host(s, perm, s->pb[s->cur_pb].pid, (size_t)s->n * 4))) return r;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+int mpm_sync(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+int mpm_get_state(mpm_solver* s, float* x, float* v, float* C, float* F, float* Jp)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    const int n = s->n, d = s->dim;
+    if (n == 0) return MPM_OK;
+    const ParticleBuffers& b = s->pb[s->cur_pb];
+    std::vector<uint32_t> pid(n);
+    int r;
+    if ((r = copy_to_host(s, pid.data(), b.pid, (size_t)n * 4))) return r;
+    struct { float* out; const float* src; int comps; } f[] = {
+        {x, b.x, d}, {v, b.v, d}, {C, b.C, d * d}, {F, b.F, d * d}, {Jp, b.Jp, 1}};
+    std::vector<std::vector<float>> tmp(5);
+    for (int i = 0; i < 5; ++i) {
+        if (!f[i].out) continue;
+        tmp[i].resize((size_t)f[i].comps * n);
+        if ((r = copy_to_host(s, tmp[i].data(), f[i].src, tmp[i].size() * sizeof(float)))) return r;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < 5; ++i) {
+        if (!f[i].out) continue;
+        for (int c = 0; c < f[i].comps; ++c) {
+            const float* src = tmp[i].data() + (size_t)c * n;
+            float* dst = f[i].out + (size_t)c * n;
+            for (int sl = 0; sl < n; ++sl) dst[pid[sl]] = src[sl];
+        }
+    }
+    return MPM_OK;
+}
+
+int mpm_set_state(mpm_solver* s, const float* x, const float* v, const float* C, const float* F,
+                  const float* Jp)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    const int n = s->n, d = s->dim;
+    if (n == 0) return MPM_OK;
+    const ParticleBuffers& b = s->pb[s->cur_pb];
+    std::vector<uint32_t> pid(n);
+    int r;
+    if ((r = copy_to_host(s, pid.data(), b.pid, (size_t)n * 4))) return r;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    struct { const float* in; float* dst; int comps; } f[] = {
+        {x, b.x, d}, {v, b.v, d}, {C, b.C, d * d}, {F, b.F, d * d}, {Jp, b.Jp, 1}};
+    std::vector<float> tmp;
+    for (auto& fi : f) {
+        if (!fi.in) continue;
+        tmp.resize((size_t)fi.comps * n);
+        for (int c = 0; c < fi.comps; ++c)
+            for (int sl = 0; sl < n; ++sl) tmp[(size_t)c * n + sl] = fi.in[(size_t)c * n + pid[sl]];
+        HIP_TRY(hipMemcpyAsync(fi.dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    // the grid no longer matches the particles
+    if ((r = reset_all_grids(s))) return r;
+    if (x) s->items_valid = false;
+    return MPM_OK;
+}
+
+int mpm_get_positions(mpm_solver* s, float* xyz)
+{
+    if (!s || !xyz) return fail(MPM_E_INVALID, "NULL argument");
+    const int n = s->n, d = s->dim;
+    std::vector<float> x((size_t)d * n);
+    int r = mpm_get_state(s, x.data(), nullptr, nullptr, nullptr, nullptr);
+    if (r) return r;
+    for (int p = 0; p < n; ++p)
+        for (int c = 0; c < d; ++c) xyz[(size_t)p * d + c] = x[(size_t)c * n + p];
+    return MPM_OK;
+}
+
+int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfric)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    const mpm_params& P = s->params;
+    const int d = s->dim;
+    const size_t nodes_alloc = (size_t)s->nblocks * 64;
+    std::vector<float4> a(nodes_alloc), c;
+    int r;
+    if ((r = copy_to_host(s, a.data(), s->ga[s->cur_grid], nodes_alloc * sizeof(float4)))) return r;
+    const bool coul = s->gvc[s->cur_grid] != nullptr;
+    if (coul && vcol) {
+        c.resize(nodes_alloc);
+        if ((r = copy_to_host(s, c.data(), s->gvc[s->cur_grid], nodes_alloc * sizeof(float4)))) return r;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    DevGrid g = s->dev_grid(s->cur_grid);
+    const int W1 = P.WIDTH + 1, H1 = P.HEIGHT + 1, D1 = d == 3 ? P.DEPTH + 1 : 1;
+    const size_t G = (size_t)W1 * H1 * D1;
+    for (int k = 0; k < D1; ++k)
+        for (int j = 0; j < H1; ++j)
+            for (int i = 0; i < W1; ++i) {
+                size_t id = ((size_t)k * H1 + j) * W1 + i;
+                int addr = d == 3 ? node_addr<3>(g, i, j, k) : node_addr<2>(g, i, j, 0);
+                const float4 q = a[addr];
+                const float comp[3] = {q.y, q.z, q.w};
+                if (mass) mass[id] = q.x;
+                for (int cc = 0; cc < d; ++cc) {
+                    if (mom) mom[cc * G + id] = comp[cc];
+                    if (vfric) vfric[cc * G + id] = comp[cc];
+                    if (vcol) {
+                        if (coul) {
+                            const float cv[3] = {c[addr].x, c[addr].y, c[addr].z};
+                            vcol[cc * G + id] = cv[cc];
+                        } else {
+                            vcol[cc * G + id] = comp[cc];
+                        }
+                    }
+                }
+            }
+    return MPM_OK;
+}
+
+int mpm_num_particles(const mpm_solver* s, int64_t* n)
+{
+    if (!s || !n) return fail(MPM_E_INVALID, "NULL argument");
+    *n = s->n;
+    return MPM_OK;
+}
+
+int mpm_set_timing(mpm_solver* s, int enabled)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    s->timing = enabled != 0;
+    return MPM_OK;
+}
+
+int mpm_reset_timers(mpm_solver* s)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    int r = collect_timers(s);
+    if (r) return r;
+    for (auto& t : s->timers) { t.total_ms = 0.0; t.launches = 0; }
+    return MPM_OK;
+}
+
+int mpm_get_kernel_time(mpm_solver* s, int kernel, double* total_ms, int64_t* launches)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (kernel < 0 || kernel >= MPM_KERNEL_COUNT) return fail(MPM_E_INVALID, "unknown kernel id");
+    int r = collect_timers(s);
+    if (r) return r;
+    if (total_ms) *total_ms = s->timers[kernel].total_ms;
+    if (launches) *launches = s->timers[kernel].launches;
+    return MPM_OK;
+}
+
+int mpm_check_finite(mpm_solver* s, int64_t* bad_particles)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    HIP_TRY(hipMemsetAsync(s->d_bad, 0, sizeof(unsigned long long), s->stream));
+    HIP_TRY(launch_check_finite(s->dim, s->dev_particles(), s->d_bad, s->stream));
+    unsigned long long bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, s->d_bad, sizeof(bad), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (bad_particles) *bad_particles = (int64_t)bad;
+    if (bad) return fail(MPM_E_NONFINITE, std::to_string(bad) + " particles hold NaN/Inf");
+    return MPM_OK;
+}
+
+int mpm_sort_pairs_host(const uint32_t* keys, int64_t n, int key_bits, uint32_t* out_keys,
+                        uint32_t* out_perm)
+{
+    if (n < 0 || n > (int64_t)INT32_MAX / 4) return fail(MPM_E_INVALID, "n out of range");
+    if (key_bits < 1 || key_bits > 32) return fail(MPM_E_INVALID, "key_bits must be in [1, 32]");
+    if (n > 0 && !keys) return fail(MPM_E_INVALID, "keys is NULL");
+    if (n == 0) return MPM_OK;
+    if (key_bits < 32)
+        for (int64_t i = 0; i < n; ++i)
+            if (keys[i] >> key_bits) return fail(MPM_E_INVALID, "key exceeds key_bits");
+    const int nn = (int)n;
+    uint32_t* buf = nullptr;
+    const size_t words = 4 * (size_t)nn + radix_scratch_words(nn);
+    HIP_TRY(hipMalloc(&buf, words * 4));
+    hipStream_t st = nullptr;
+    int rc = MPM_OK;
+    do {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { rc = fail(MPM_E_HIP, "stream"); break; }
+        uint32_t *k = buf, *v = buf + nn, *ka = buf + 2 * (size_t)nn, *va = buf + 3 * (size_t)nn;
+        uint32_t* scratch = buf + 4 * (size_t)nn;
+        if (hipMemcpyAsync(k, keys, (size_t)nn * 4, hipMemcpyHostToDevice, st) != hipSuccess) { rc = fail(MPM_E_HIP, "upload"); break; }
+        bool in_alt = false;
+        hipError_t e = radix_sort_pairs(k, v, ka, va, nn, key_bits, true, scratch, st, &in_alt);
+        if (e != hipSuccess) { rc = fail(MPM_E_HIP, std::string("radix sort: ") + hipGetErrorString(e)); break; }
+        if (out_keys && hipMemcpyAsync(out_keys, in_alt ? ka : k, (size_t)nn * 4, hipMemcpyDeviceToHost, st) != hipSuccess) { rc = fail(MPM_E_HIP, "download"); break; }
+        if (out_perm && hipMemcpyAsync(out_perm, in_alt ? va : v, (size_t)nn * 4, hipMemcpyDeviceToHost, st) != hipSuccess) { rc = fail(MPM_E_HIP, "download"); break; }
+        if (hipStreamSynchronize(st) != hipSuccess) { rc = fail(MPM_E_HIP, "sync"); break; }
+    } while (0);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    return rc;
+}
+
+const char* mpm_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+// mpm_device.hpp — device-side math of the MLS-MPM substep for gfx950.
+//
+// B-spline weights, the reference's SVDs (2x2 analytic, 3x3 cyclic Jacobi),
+// Kirchhoff stress with hardening, and the block-major grid addressing used by
+// every kernel.  All arithmetic is FP32, matrices are register arrays in row-major
+// order, and every loop is fully unrolled so they stay in VGPRs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpm/params.h"
+
+namespace mpm {
+
+// ---------------------------------------------------------------------------
+// Parameters as the kernels consume them (derived once on the host).
+struct Phys {
+    float dx, inv_dx, dt, Dinv;      // Dinv = D^-1: 3/dx^2 cubic, 4/dx^2 quadratic
+    float gravity, friction, margin; // margin in cells (BORDER_MARGIN)
+    float theta0, theta1;            // 1 - THETA_C, 1 + THETA_S
+    float xi, mu, lambda;
+    float mass_eps;
+    int   W, H, D;                   // cells per axis; nodes 0..W
+    int   boundary;
+};
+
+// Grid: block-major storage, 64 nodes per block (4x4x4 in 3D, 8x8 in 2D), one
+// float4 per node: (mass, momentum) after P2G, (mass, velocity) after the grid
+// update.  vc holds the collision velocity when it differs from the friction
+// velocity (Coulomb walls); flags marks the blocks a scatter touched.
+struct DevGrid {
+    float4*   a;
+    float4*   vc;
+    uint32_t* flags;
+    int       nbx, nby, nbz;
+};
+
+struct DevParticles {
+    float*       x;     // [DIM][n]
+    float*       v;     // [DIM][n]
+    float*       C;     // [DIM*DIM][n]
+    float*       F;     // [DIM*DIM][n]
+    float*       Jp;    // [n]
+    const float* mass;  // [n]
+    const float* vol;   // [n]
+    uint32_t*    pid;   // [n] original particle index of each slot
+    int          n;
+};
+
+template <int DIM> struct BlockShape;
+template <> struct BlockShape<2> { static constexpr int BS = 8; static constexpr int SHIFT = 3; };
+template <> struct BlockShape<3> { static constexpr int BS = 4; static constexpr int SHIFT = 2; };
+
+template <int DIM>
+__host__ __device__ inline int node_addr(const DevGrid& g, int i, int j, int k)
+{
+    constexpr int S = BlockShape<DIM>::SHIFT;
+    constexpr int M = (1 << S) - 1;
+    if constexpr (DIM == 2) {
+        int blk = (j >> S) * g.nbx + (i >> S);
+        return (blk << 6) | ((j & M) << S) | (i & M);
+    } else {
+        int blk = ((k >> S) * g.nby + (j >> S)) * g.nbx + (i >> S);
+        return (blk << 6) | ((k & M) << (2 * S)) | ((j & M) << S) | (i & M);
+    }
+}
+
+template <int DIM>
+__host__ __device__ inline int block_of(const DevGrid& g, int i, int j, int k)
+{
+    constexpr int S = BlockShape<DIM>::SHIFT;
+    if constexpr (DIM == 2) return (j >> S) * g.nbx + (i >> S);
+    else return ((k >> S) * g.nby + (j >> S)) * g.nbx + (i >> S);
+}
+
+// ---------------------------------------------------------------------------
+// B-spline weights along one axis (shaders/2d/common.glsl:43-55 for cubic).
+template <int ORDER> struct Spline;
+
+template <> struct Spline<3> {
+    static constexpr int W = 4;
+    __device__ static inline int base(float xg) { return (int)xg - 1; }   // int() truncation
+    __device__ static inline float N(float x)
+    {
+        float a = fabsf(x);
+        if (a < 1.0f) return 0.5f * (a * a * a) - (a * a) + (2.0f / 3.0f);
+        if (a < 2.0f) { float t = 2.0f - a; return t * t * t / 6.0f; }
+        return 0.0f;
+    }
+};
+
+template <> struct Spline<2> {
+    static constexpr int W = 3;
+    __device__ static inline int base(float xg) { return (int)floorf(xg - 0.5f); }
+    __device__ static inline float N(float x)
+    {
+        float a = fabsf(x);
+        if (a < 0.5f) return 0.75f - a * a;
+        if (a < 1.5f) { float t = 1.5f - a; return 0.5f * t * t; }
+        return 0.0f;
+    }
+};
+
+// Per-particle stencil: base node, per-axis weights and grid-unit distances
+// dist = xg - node (the reference's pPos - gi).
+template <int DIM, int ORDER>
+struct Stencil {
+    static constexpr int W = Spline<ORDER>::W;
+    int   base[DIM];
+    float w[DIM][W];
+    float d[DIM][W];
+    __device__ inline void init(const float* xg)
+    {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            base[a] = Spline<ORDER>::base(xg[a]);
+#pragma unroll
+            for (int s = 0; s < W; ++s) {
+                d[a][s] = xg[a] - (float)(base[a] + s);
+                w[a][s] = Spline<ORDER>::N(d[a][s]);
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Small matrix helpers (row-major, fully unrolled).
+template <int DIM>
+__device__ inline float det(const float* A)
+{
+    if constexpr (DIM == 2) return A[0] * A[3] - A[2] * A[1];
+    else
+        return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+               A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+template <int DIM>
+__device__ inline void matmul(const float* A, const float* B, float* C)
+{
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            float s = A[r * DIM] * B[c];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) s += A[r * DIM + k] * B[k * DIM + c];
+            C[r * DIM + c] = s;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// SVD, the reference's algorithms and conventions: sigma unsorted, det U > 0 by
+// flipping the last column of U and the last sigma.
+//
+// 2x2: shaders/2d/common.glsl:2-37 (eigenvectors of M^T M from one angle).
+__device__ inline void svd2(const float* M, float* U, float* sig, float* V)
+{
+    float a = M[0] * M[0] + M[2] * M[2];
+    float b = M[0] * M[1] + M[2] * M[3];
+    float d = M[1] * M[1] + M[3] * M[3];
+    float theta = (fabsf(b) < 1e-10f && fabsf(a - d) < 1e-10f) ? 0.0f
+                                                               : 0.5f * atan2f(2.0f * b, a - d);
+    float st, ct;
+    sincosf(theta, &st, &ct);
+    V[0] = ct; V[1] = -st; V[2] = st; V[3] = ct;
+    // diag(V^T (M^T M) V)
+    float t00 = ct * a + st * b, t01 = ct * b + st * d;
+    float t10 = -st * a + ct * b, t11 = -st * b + ct * d;
+    float d0 = t00 * ct + t01 * st;
+    float d1 = -t10 * st + t11 * ct;
+    sig[0] = sqrtf(fmaxf(d0, 1e-12f));
+    sig[1] = sqrtf(fmaxf(d1, 1e-12f));
+    float i0 = 1.0f / sig[0], i1 = 1.0f / sig[1];
+    U[0] = (M[0] * ct + M[1] * st) * i0;
+    U[1] = (-M[0] * st + M[1] * ct) * i1;
+    U[2] = (M[2] * ct + M[3] * st) * i0;
+    U[3] = (-M[2] * st + M[3] * ct) * i1;
+    if (U[0] * U[3] - U[2] * U[1] < 0.0f) {
+        U[1] = -U[1]; U[3] = -U[3]; sig[1] = -sig[1];
+    }
+}
+
+// 3x3: shaders/3d/common.glsl:8-105 — cyclic Jacobi eigen-decomposition of the
+// symmetric S = M^T M (Golub & Van Loan, Matrix Computations, sec. 8.5: symmetric
+// Schur pair, six sweeps of (0,1), (0,2), (1,2)), then U = M V diag(1/sigma).
+__device__ inline void jacobi_pair(float& spp, float& sqq, float& spq, float& sr_p, float& sr_q,
+                                   float* V, int p, int q)
+{
+    // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]]
+    float c = 1.0f, s = 0.0f;
+    if (!(fabsf(spq) < 1e-10f)) {
+        float tau = (sqq - spp) / (2.0f * spq);
+        float stt = sqrtf(1.0f + tau * tau);
+        float t = 1.0f / (fabsf(tau) + stt);
+        if (tau < 0.0f) t = -t;
+        c = 1.0f / sqrtf(1.0f + t * t);
+        s = t * c;
+    }
+    float Spp = spp, Sqq = sqq, Spq = spq;
+    spp = c * c * Spp - 2.0f * s * c * Spq + s * s * Sqq;
+    sqq = s * s * Spp + 2.0f * s * c * Spq + c * c * Sqq;
+    spq = 0.0f;
+    // the remaining off-diagonal pair (r,p), (r,q) with r the third index
+    float Srp = sr_p, Srq = sr_q;
+    sr_p = c * Srp - s * Srq;
+    sr_q = s * Srp + c * Srq;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float vp = V[r * 3 + p], vq = V[r * 3 + q];
+        V[r * 3 + p] = c * vp - s * vq;
+        V[r * 3 + q] = s * vp + c * vq;
+    }
+}
+
+__device__ inline void svd3(const float* M, float* U, float* sig, float* V)
+{
+    float s00 = M[0] * M[0] + M[3] * M[3] + M[6] * M[6];
+    float s01 = M[0] * M[1] + M[3] * M[4] + M[6] * M[7];
+    float s02 = M[0] * M[2] + M[3] * M[5] + M[6] * M[8];
+    float s11 = M[1] * M[1] + M[4] * M[4] + M[7] * M[7];
+    float s12 = M[1] * M[2] + M[4] * M[5] + M[7] * M[8];
+    float s22 = M[2] * M[2] + M[5] * M[5] + M[8] * M[8];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int sweep = 0; sweep < 6; ++sweep) {
+        jacobi_pair(s00, s11, s01, s02, s12, V, 0, 1);   // third index 2: (2,0), (2,1)
+        jacobi_pair(s00, s22, s02, s01, s12, V, 0, 2);   // third index 1: (1,0), (1,2)
+        jacobi_pair(s11, s22, s12, s01, s02, V, 1, 2);   // third index 0: (0,1), (0,2)
+    }
+    sig[0] = sqrtf(fmaxf(s00, 1e-12f));
+    sig[1] = sqrtf(fmaxf(s11, 1e-12f));
+    sig[2] = sqrtf(fmaxf(s22, 1e-12f));
+    float inv[3] = {1.0f / sig[0], 1.0f / sig[1], 1.0f / sig[2]};
+    float MV[9];
+    matmul<3>(M, V, MV);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U[r * 3 + c] = MV[r * 3 + c] * inv[c];
+    if (det<3>(U) < 0.0f) {
+        U[2] = -U[2]; U[5] = -U[5]; U[8] = -U[8];
+        sig[2] = -sig[2];
+    }
+}
+
+template <int DIM>
+__device__ inline void svd(const float* M, float* U, float* sig, float* V)
+{
+    if constexpr (DIM == 2) svd2(M, U, sig, V);
+    else svd3(M, U, sig, V);
+}
+
+// Kirchhoff stress of F with polar rotation R = U V^T (shaders/2d/p2g.glsl:44-59):
+//   tau = 2 mu (F - R) F^T + lambda (J - 1) J I,  mu, lambda scaled by exp(XI (1 - Jp)).
+template <int DIM>
+__device__ inline void kirchhoff(const Phys& P, const float* F, const float* U, const float* V,
+                                 float J, float Jp, float* tau)
+{
+    float Rm[DIM * DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            float s = U[r * DIM] * V[c * DIM];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) s += U[r * DIM + k] * V[c * DIM + k];
+            Rm[r * DIM + c] = s;
+        }
+    float harden = expf(P.xi * (1.0f - Jp));
+    float tmu = 2.0f * (P.mu * harden);
+    float la = P.lambda * harden;
+    float A[DIM * DIM];
+#pragma unroll
+    for (int i = 0; i < DIM * DIM; ++i) A[i] = tmu * (F[i] - Rm[i]);
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            float s = A[r * DIM] * F[c * DIM];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) s += A[r * DIM + k] * F[c * DIM + k];
+            tau[r * DIM + c] = s;
+        }
+    float sI = la * (J - 1.0f) * J;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) tau[i * DIM + i] += sI;
+}
+
+}  // namespace mpm

@@ -1,0 +1,30 @@
+// mpm_sort.hpp — launchers of the binning kernels (mpm_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "mpm_device.hpp"
+
+namespace mpm {
+
+// uint32 words of scratch radix_sort_pairs needs for n keys.
+size_t radix_scratch_words(int n);
+
+// Stable LSD radix sort of n (key, val) pairs over the low key_bits bits.  Ping-pongs
+// between (keys, vals) and (keys_alt, vals_alt); *result_in_alt says where the
+// sorted pairs ended.  init_vals: fill vals with 0..n-1 first.
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                            int n, int key_bits, bool init_vals, uint32_t* scratch, hipStream_t st,
+                            bool* result_in_alt);
+
+hipError_t launch_cell_keys(int dim, const float* x, int n, const Phys& ph, const DevGrid& g,
+                            uint32_t* keys, hipStream_t st);
+hipError_t launch_gather_fields(const float* src, float* dst, const uint32_t* perm, int n, int ncomp,
+                                hipStream_t st);
+hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t* perm, int n,
+                             hipStream_t st);
+hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int* n_items,
+                             int max_items, hipStream_t st);
+
+}  // namespace mpm

@@ -1,0 +1,179 @@
+"""Host-side mirror of the reference's solver interface over the C-ABI.
+
+The reference selects ``GPUSolver`` or ``CPUSolver`` at compile time and uses them
+through ``Solver()``, ``Step()`` and ``GetParticlePositions()`` (src/main.cpp:22-29,
+src/gpu/GPUSolver.h:10-17, src/cpu/CPUSolver.h:10-27).  ``GPUSolver`` here keeps
+those names and meanings on top of libmpm_hip.so, and adds the stage-level calls
+(``p2g``, ``grid_update``, ``g2p``, ``reset``), state and grid readback, binning and
+timing.  Errors raise ``MPMError`` (the reference throws from its constructor,
+GPUSolver.cpp:39-50).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import CParticles, MPMError, check
+from .params import COptions, Options, Params, reference_params
+from .scenes import ParticleSet, reference_snowballs
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, np.float32))
+    if a.size != int(np.prod(shape)):
+        raise ValueError(f"expected {shape}, got {a.shape}")
+    return a.reshape(shape)
+
+
+class GPUSolver:
+    """MLS-MPM snow solver on the GPU (drop-in for the reference's GPUSolver)."""
+
+    def __init__(self, params: Params | None = None, particles: ParticleSet | None = None,
+                 options: Options | None = None, C=None, F=None, Jp=None):
+        lib = _lib.load()
+        self.params = params if params is not None else reference_params(2)
+        if particles is None:
+            particles = reference_snowballs(self.params.dim)
+        d = self.params.dim
+        if particles.dim != d:
+            raise ValueError("particle dimension does not match params.dim")
+        n = particles.n
+        self.n = n
+        self.dim = d
+        self._keep = [_f32(particles.x, (d, n)), _f32(particles.v, (d, n)),
+                      _f32(particles.mass, (n,)), _f32(particles.vol, (n,))]
+        extra = [None if C is None else _f32(C, (d * d, n)),
+                 None if F is None else _f32(F, (d * d, n)),
+                 None if Jp is None else _f32(Jp, (n,))]
+        cp = CParticles(n, _ptr(self._keep[0]), _ptr(self._keep[1]), _ptr(self._keep[2]),
+                        _ptr(self._keep[3]), _ptr(extra[0]), _ptr(extra[1]), _ptr(extra[2]))
+        opts = COptions.of(options or Options())
+        h = ctypes.c_void_p()
+        check(lib.mpm_create(ctypes.byref(self.params), ctypes.byref(cp), ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self._cached_positions = np.zeros((n, d), np.float32)
+
+    # -- lifetime
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference's interface
+    def Step(self, n_substeps: int = 1) -> None:
+        """GPUSolver::Step (GPUSolver.cpp:63-80): one (or n) substeps, asynchronous."""
+        check(self._lib.mpm_step(self._h, int(n_substeps)))
+
+    def GetParticlePositions(self) -> np.ndarray:
+        """GPUSolver::GetParticlePositions (GPUSolver.cpp:276-295): synchronises and
+        returns an (n, dim) array owned by the solver, valid until the next call."""
+        check(self._lib.mpm_get_positions(self._h, _ptr(self._cached_positions)))
+        return self._cached_positions
+
+    step = Step
+
+    # -- stages (one per dispatch of GPUSolver::Step)
+    def p2g(self) -> None:
+        check(self._lib.mpm_p2g(self._h))
+
+    def grid_update(self) -> None:
+        check(self._lib.mpm_grid_update(self._h))
+
+    def g2p(self) -> None:
+        check(self._lib.mpm_g2p(self._h))
+
+    def reset(self) -> None:
+        check(self._lib.mpm_reset_grid(self._h))
+
+    def sort(self) -> None:
+        check(self._lib.mpm_sort(self._h))
+
+    def sync(self) -> None:
+        check(self._lib.mpm_sync(self._h))
+
+    # -- state
+    def get_state(self) -> dict:
+        d, n = self.dim, self.n
+        x = np.empty((d, n), np.float32)
+        v = np.empty((d, n), np.float32)
+        C = np.empty((d * d, n), np.float32)
+        F = np.empty((d * d, n), np.float32)
+        Jp = np.empty(n, np.float32)
+        check(self._lib.mpm_get_state(self._h, _ptr(x), _ptr(v), _ptr(C), _ptr(F), _ptr(Jp)))
+        return {"x": x, "v": v, "C": C, "F": F, "Jp": Jp}
+
+    def set_state(self, x=None, v=None, C=None, F=None, Jp=None) -> None:
+        d, n = self.dim, self.n
+        arrs = [None if x is None else _f32(x, (d, n)), None if v is None else _f32(v, (d, n)),
+                None if C is None else _f32(C, (d * d, n)), None if F is None else _f32(F, (d * d, n)),
+                None if Jp is None else _f32(Jp, (n,))]
+        check(self._lib.mpm_set_state(self._h, *[_ptr(a) for a in arrs]))
+
+    def get_grid(self) -> dict:
+        G = self.params.num_nodes
+        d = self.dim
+        m = np.empty(G, np.float32)
+        mom = np.empty((d, G), np.float32)
+        vc = np.empty((d, G), np.float32)
+        vf = np.empty((d, G), np.float32)
+        check(self._lib.mpm_get_grid(self._h, _ptr(m), _ptr(mom), _ptr(vc), _ptr(vf)))
+        return {"mass": m, "mom": mom, "vcol": vc, "vfric": vf}
+
+    def get_sort(self) -> tuple[np.ndarray, np.ndarray]:
+        keys = np.empty(self.n, np.uint32)
+        perm = np.empty(self.n, np.uint32)
+        check(self._lib.mpm_get_sort(self._h, _ptr(keys), _ptr(perm)))
+        return keys, perm
+
+    def check_finite(self) -> int:
+        bad = ctypes.c_int64(0)
+        code = self._lib.mpm_check_finite(self._h, ctypes.byref(bad))
+        if code not in (_lib.MPM_OK, _lib.MPM_E_NONFINITE):
+            check(code)
+        return int(bad.value)
+
+    # -- timing
+    def set_timing(self, on: bool = True) -> None:
+        check(self._lib.mpm_set_timing(self._h, int(on)))
+
+    def reset_timers(self) -> None:
+        check(self._lib.mpm_reset_timers(self._h))
+
+    def kernel_time(self, kernel: int) -> tuple[float, int]:
+        ms = ctypes.c_double(0.0)
+        k = ctypes.c_int64(0)
+        check(self._lib.mpm_get_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(k)))
+        return float(ms.value), int(k.value)
+
+
+def sort_pairs(keys: np.ndarray, key_bits: int = 32) -> tuple[np.ndarray, np.ndarray]:
+    """Stable device sort of uint32 keys; returns (sorted keys, permutation)."""
+    lib = _lib.load()
+    k = np.ascontiguousarray(keys, np.uint32)
+    n = k.size
+    ok = np.empty(n, np.uint32)
+    op = np.empty(n, np.uint32)
+    check(lib.mpm_sort_pairs_host(_ptr(k), n, int(key_bits), _ptr(ok), _ptr(op)))
+    return ok, op
+
+
+__all__ = ["GPUSolver", "MPMError", "sort_pairs"]

@@ -1,0 +1,200 @@
+"""HIP path against the oracle, through the C-ABI (libmpm_hip.so).
+
+Tolerances (FP32 physics; the GPU sums its P2G with float atomics in an order
+that differs from the oracle's particle order, and fuses multiply-adds):
+  * one stage on identical input: relative 2e-5 of the field's largest value;
+  * N substeps of a whole scene: x within 1e-3 cells, v and C within 1e-3 of the
+    field's scale, F_E and Jp within 1e-3 (max over particles), medians 100x tighter.
+Errors against the FP64 twin of the oracle are printed next to the FP32 ones.
+"""
+import numpy as np
+import pytest
+
+from mpm_snow_sim2d_amd.params import Options
+from tests.helpers import CASES, cell_keys, make_case, state_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(P, ps, **kw):
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    return GPUSolver(P, ps, Options(**kw))
+
+
+def _oracle_state(orc, ps, dtype):
+    return orc.OracleState.from_particles(ps, dtype)
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------------ stage by stage
+@pytest.mark.parametrize("case", list(CASES))
+def test_stage_p2g(orc, case):
+    P, ps = make_case(case)
+    _flush(f"[p2g] {case} n={ps.n}")
+    s = _solver(P, ps, fused=0)
+    s.p2g()
+    g = s.get_grid()
+    st = _oracle_state(orc, ps, np.float32)
+    gm, gmom = orc.p2g(P, st)
+    assert np.abs(g["mass"] - gm).max() <= 2e-5 * gm.max()
+    for d in range(P.dim):
+        scale = max(np.abs(gmom[d]).max(), 1e-6 * np.abs(gmom).max())
+        assert np.abs(g["mom"][d] - gmom[d]).max() <= 2e-5 * scale
+    s.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_stage_grid_update_and_g2p(orc, case):
+    """grid_update on the GPU's own P2G grid equals the oracle's grid update of that
+    same grid; G2P from those velocities equals the oracle's G2P."""
+    P, ps = make_case(case)
+    _flush(f"[gu/g2p] {case} n={ps.n}")
+    s = _solver(P, ps, fused=0)
+    s.p2g()
+    g0 = s.get_grid()
+    s.grid_update()
+    g1 = s.get_grid()
+    vc, vf = orc.grid_update(P, g0["mass"], g0["mom"])
+    sc = max(np.abs(vf).max(), 1e-30)
+    assert np.abs(g1["vfric"] - vf).max() <= 2e-5 * sc
+    assert np.abs(g1["vcol"] - vc).max() <= 2e-5 * sc
+    s.g2p()
+    gpu = s.get_state()
+    st = _oracle_state(orc, ps, np.float32)
+    orc.g2p(P, st, g1["vcol"], g1["vfric"])
+    ref = orc.to_solver_state(P, st)
+    err = state_errors(gpu, ref, P.dx)
+    _flush(f"    g2p errors {err}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert err[k][0] < 2e-5, (k, err[k])
+    s.close()
+
+
+# ------------------------------------------------------------------ whole substeps
+@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("case", list(CASES))
+def test_substeps_match_oracle(orc, case, fused):
+    P, ps = make_case(case)
+    nsub = 20
+    _flush(f"[substeps] {case} fused={fused} n={ps.n} steps={nsub}")
+    s = _solver(P, ps, fused=fused, sort_every=8)
+    s.Step(nsub)
+    gpu = s.get_state()
+    assert s.check_finite() == 0
+    s.close()
+    o32 = orc.run(P, _oracle_state(orc, ps, np.float32), nsub)
+    o64 = orc.run(P, _oracle_state(orc, ps, np.float64), nsub)
+    e32 = state_errors(gpu, orc.to_solver_state(P, o32), P.dx)
+    e64 = state_errors(gpu, orc.to_solver_state(P, o64), P.dx)
+    o32v64 = state_errors(orc.to_solver_state(P, o32), orc.to_solver_state(P, o64), P.dx)
+    _flush(f"    gpu-vs-f32 {e32}\n    gpu-vs-f64 {e64}\n    f32-vs-f64 {o32v64}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert e32[k][0] < 1e-3, (k, e32[k])
+        assert e32[k][1] < 1e-5, (k, e32[k])
+
+
+@pytest.mark.parametrize("case", ["snow2d_quad_sticky", "ref2d_cubic_coulomb"])
+def test_longer_run_fused_vs_staged(orc, case):
+    """200 substeps: fused and staged pipelines agree with each other and with the
+    FP32 oracle in aggregate (centre of mass, momentum, mean Jp)."""
+    P, ps = make_case(case)
+    nsub = 200
+    _flush(f"[long] {case} n={ps.n} steps={nsub}")
+    outs = []
+    for fused in (0, 1):
+        s = _solver(P, ps, fused=fused, sort_every=16)
+        s.Step(nsub)
+        outs.append(s.get_state())
+        s.close()
+    o32 = orc.to_solver_state(P, orc.run(P, _oracle_state(orc, ps, np.float32), nsub))
+    m = ps.mass.astype(np.float64)
+    for st in outs:
+        com = (st["x"] * m).sum(1) / m.sum()
+        com_o = (o32["x"] * m).sum(1) / m.sum()
+        assert np.abs(com - com_o).max() < 1e-3 * P.dx * 10
+        mom = (st["v"] * m).sum(1)
+        mom_o = (o32["v"] * m).sum(1)
+        assert np.abs(mom - mom_o).max() <= 1e-3 * np.abs(m[None] * np.abs(o32["v"])).sum()
+        assert abs(st["Jp"].mean() - o32["Jp"].mean()) < 1e-4
+
+
+# ------------------------------------------------------------------ binning
+@pytest.mark.parametrize("case", ["snow3d_quad_sticky", "ref2d_cubic_coulomb", "snow2d_quad_sticky"])
+def test_sort_bit_exact(case):
+    P, ps = make_case(case)
+    _flush(f"[sort] {case} n={ps.n}")
+    s = _solver(P, ps)
+    s.sort()
+    keys, perm = s.get_sort()
+    want = cell_keys(P, ps.x)
+    order = np.argsort(want, kind="stable")
+    assert np.array_equal(keys, want[order])
+    assert np.array_equal(perm, order.astype(np.uint32))
+    # the reordered state still reads back in the caller's order
+    st = s.get_state()
+    assert np.array_equal(st["x"], ps.x.astype(np.float32))
+    s.close()
+
+
+@pytest.mark.parametrize("n,bits", [(0, 8), (1, 8), (7, 3), (4096, 12), (4097, 12), (12345, 24),
+                                    (100_000, 32), (300_001, 17)])
+def test_sort_pairs_edge_cases(n, bits):
+    from mpm_snow_sim2d_amd.solver import sort_pairs
+    rng = np.random.default_rng(n + bits)
+    hi = (1 << bits) if bits < 32 else (1 << 32)
+    keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    if n > 10:
+        keys[: n // 3] = keys[0]          # long run of equal keys: stability matters
+    _flush(f"[sort_pairs] n={n} bits={bits}")
+    k, p = sort_pairs(keys, bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(p, order.astype(np.uint32))
+
+
+def test_sort_pairs_rejects_oversized_keys():
+    from mpm_snow_sim2d_amd.solver import MPMError, sort_pairs
+    with pytest.raises(MPMError):
+        sort_pairs(np.array([1, 300], np.uint32), 8)
+
+
+# ------------------------------------------------------------------ interface behaviour
+def test_reference_interface_and_state_roundtrip():
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("ref2d_cubic_coulomb")
+    s = GPUSolver(P, ps)
+    pos = s.GetParticlePositions()
+    assert pos.shape == (ps.n, 2) and np.array_equal(pos, ps.x.T.astype(np.float32))
+    s.Step()
+    s.Step(3)
+    st = s.get_state()
+    s.set_state(**st)
+    st2 = s.get_state()
+    for k in st:
+        assert np.array_equal(st[k], st2[k])
+    s.close()
+
+
+def test_nonfinite_state_is_reported():
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("snow2d_quad_sticky")
+    s = GPUSolver(P, ps)
+    st = s.get_state()
+    st["v"][0, 5] = np.nan
+    st["F"][0, 9] = np.inf
+    s.set_state(**st)
+    assert s.check_finite() == 2
+    s.close()
+
+
+def test_empty_solver():
+    from mpm_snow_sim2d_amd.scenes import ParticleSet
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, _ = make_case("snow2d_quad_sticky")
+    s = GPUSolver(P, ParticleSet.empty(2))
+    s.Step(3)
+    assert s.GetParticlePositions().shape == (0, 2)
+    s.close()
