@@ -264,8 +264,11 @@ int do_sort(mpm_solver* s)
         e = launch_gather_u32(src.pid, dst.pid, perm, n, s->stream);
         if (e != hipSuccess) return e;
         s->cur_pb = 1 - s->cur_pb;
+        // the pair of key/value buffers the sort did not end in is free now
+        uint32_t* tmp_a = in_alt ? s->keys : s->keys_alt;
+        uint32_t* tmp_b = in_alt ? s->vals : s->vals_alt;
         return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->n_items, s->max_items,
-                                 s->stream);
+                                 tmp_a, tmp_b, s->radix_scratch, s->stream);
     };
     int r = timed(s, MPM_KERNEL_SORT, body);
     if (r) return r;

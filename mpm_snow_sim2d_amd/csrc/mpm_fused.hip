@@ -47,6 +47,31 @@ __device__ inline bool in_grid(const Phys& ph, int i, int j, int k)
     return ok;
 }
 
+// Element a (0 <= a < W) of a register array, through selects (no scratch).
+template <int W>
+__device__ inline float pick(const float (&arr)[W], int a)
+{
+    float r = arr[0];
+#pragma unroll
+    for (int s = 1; s < W; ++s) r = (a == s) ? arr[s] : r;
+    return r;
+}
+
+// P2G contribution of one particle to one node: (w m, w (m v + A (x_i - x_p))).
+template <int DIM>
+__device__ inline void node_contrib(float w, float pm, const float* mvp, const float* A, const float* xi,
+                                    float* vals)
+{
+    vals[0] = w * pm;
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+        float s = mvp[r];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) s += A[r * DIM + c] * xi[c];
+        vals[1 + r] = w * s;
+    }
+}
+
 template <int DIM, int ORDER, bool COULOMB, bool G2P>
 __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevGrid dst, TileWork work,
                                                Phys ph)
@@ -266,43 +291,68 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
                 rb[a] = st.base[a] - o[a];
                 tile_ok &= rb[a] >= 0 && rb[a] + W <= T;
             }
-#pragma unroll
-            for (int a0 = 0; a0 < W; ++a0)
-#pragma unroll
-                for (int a1 = 0; a1 < W; ++a1)
-#pragma unroll
-                    for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
-                        int idx[3] = {st.base[0] + a0, st.base[1] + a1, DIM == 3 ? st.base[DIM - 1] + a2 : 0};
-                        if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
+            if (tile_ok) {
+                // LDS scatter in a per-lane rotated node order: lanes that hold
+                // particles of the same cell (neighbours after binning) start at
+                // different stencil nodes, so one ds_add never sees several lanes on
+                // one address.
+                constexpr int NS = DIM == 3 ? W * W * W : W * W;
+                int r0 = (tid & 63) % NS;
+                int a2 = DIM == 3 ? r0 % W : 0;
+                int a1 = DIM == 3 ? (r0 / W) % W : r0 % W;
+                int a0 = DIM == 3 ? r0 / (W * W) : r0 / W;
+#pragma unroll 3
+                for (int sn = 0; sn < NS; ++sn) {
+                    int idx[3] = {st.base[0] + a0, st.base[1] + a1, DIM == 3 ? st.base[DIM - 1] + a2 : 0};
+                    if (in_grid<DIM>(ph, idx[0], idx[1], idx[2])) {
                         float xi[DIM];
-                        xi[0] = -st.d[0][a0] * ph.dx;
-                        xi[1] = -st.d[1][a1] * ph.dx;
-                        float w = st.w[0][a0] * st.w[1][a1];
+                        xi[0] = -pick<W>(st.d[0], a0) * ph.dx;
+                        xi[1] = -pick<W>(st.d[1], a1) * ph.dx;
+                        float w = pick<W>(st.w[0], a0) * pick<W>(st.w[1], a1);
                         if constexpr (DIM == 3) {
-                            xi[DIM - 1] = -st.d[DIM - 1][a2] * ph.dx;
-                            w = w * st.w[DIM - 1][a2];
+                            xi[DIM - 1] = -pick<W>(st.d[DIM - 1], a2) * ph.dx;
+                            w = w * pick<W>(st.w[DIM - 1], a2);
                         }
                         float vals[DIM + 1];
-                        vals[0] = w * pm;
+                        node_contrib<DIM>(w, pm, mvp, A, xi, vals);
+                        int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                         : (rb[1] + a1) * T + (rb[0] + a0);
 #pragma unroll
-                        for (int r = 0; r < DIM; ++r) {
-                            float s = mvp[r];
+                        for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc[c][e], vals[c]);
+                    }
+                    if constexpr (DIM == 3) {
+                        if (++a2 == W) { a2 = 0; if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; } }
+                    } else {
+                        if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; }
+                    }
+                }
+            } else {
+                // stencil outside the tile (particle drifted since binning): direct
+                // global atomics
 #pragma unroll
-                            for (int c = 0; c < DIM; ++c) s += A[r * DIM + c] * xi[c];
-                            vals[1 + r] = w * s;
-                        }
-                        if (tile_ok) {
-                            int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                             : (rb[1] + a1) * T + (rb[0] + a0);
+                for (int a0 = 0; a0 < W; ++a0)
 #pragma unroll
-                            for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc[c][e], vals[c]);
-                        } else {
+                    for (int a1 = 0; a1 < W; ++a1)
+#pragma unroll
+                        for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                            int idx[3] = {st.base[0] + a0, st.base[1] + a1, DIM == 3 ? st.base[DIM - 1] + a2 : 0};
+                            if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
+                            float xi[DIM];
+                            xi[0] = -st.d[0][a0] * ph.dx;
+                            xi[1] = -st.d[1][a1] * ph.dx;
+                            float w = st.w[0][a0] * st.w[1][a1];
+                            if constexpr (DIM == 3) {
+                                xi[DIM - 1] = -st.d[DIM - 1][a2] * ph.dx;
+                                w = w * st.w[DIM - 1][a2];
+                            }
+                            float vals[DIM + 1];
+                            node_contrib<DIM>(w, pm, mvp, A, xi, vals);
                             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
 #pragma unroll
                             for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
                             dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
                         }
-                    }
+            }
         }
         __syncthreads();
 

@@ -351,31 +351,52 @@ __global__ void __launch_bounds__(256) k_g2p(DevParticles P, DevGrid g, Phys ph)
     P.Jp[p] = P.Jp[p] * ratio;
 }
 
-// Fused-pipeline grid pass: per block of the chip-wide flag arrays, converts the
-// touched blocks of `cur` to velocities and clears the touched blocks of `old`
-// (the grid the last G2P read) so the next scatter starts from zero.
-template <int DIM, bool COULOMB>
-__global__ void __launch_bounds__(64) k_grid_update_clear(DevGrid cur, DevGrid old, Phys ph)
+// Fused-pipeline grid pass: converts the touched blocks of `cur` to velocities
+// and clears the touched blocks of `old` (the grid the last G2P read) so the next
+// scatter starts from zero.  One workgroup scans 64 consecutive block flags with one
+// coalesced load, compacts the touched ones in LDS, and its four waves then take one
+// 64-node block each — the launch is nblocks/64 workgroups, not one per block.
+__device__ inline uint64_t lanemask_lt64()
 {
-    int blk = blockIdx.x;
-    uint32_t fc = cur.flags[blk], fo = old.flags[blk];
-    if ((fc | fo) == 0u) return;
-    int lane = threadIdx.x;
-    int addr = (blk << 6) | lane;
-    if (fo) {
-        old.a[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (COULOMB) old.vc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane == 0) old.flags[blk] = 0u;
+    int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+template <int DIM, bool COULOMB>
+__global__ void __launch_bounds__(256) k_grid_update_clear(DevGrid cur, DevGrid old, Phys ph, int nblocks)
+{
+    __shared__ int s_list[64];
+    __shared__ int s_cnt;
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        int b = blockIdx.x * 64 + tid;
+        int f = 0;
+        if (b < nblocks) f = (cur.flags[b] ? 1 : 0) | (old.flags[b] ? 2 : 0);
+        uint64_t m = __ballot(f != 0);
+        if (f) s_list[__popcll(m & lanemask_lt64())] = (b << 2) | f;
+        if (tid == 0) s_cnt = __popcll(m);
     }
-    if (fc) {
-        int i, j, k;
-        block_node<DIM>(cur, blk, lane, i, j, k);
-        float4 a = cur.a[addr];
-        float vf[DIM], vc[DIM];
-        node_update<DIM, COULOMB>(ph, i, j, k, a, vf, vc);
-        cur.a[addr] = make_float4(a.x, vf[0], vf[1], DIM == 3 ? vf[DIM - 1] : 0.f);
-        if constexpr (COULOMB)
-            cur.vc[addr] = make_float4(vc[0], vc[1], DIM == 3 ? vc[DIM - 1] : 0.f, 0.f);
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int q = wave; q < s_cnt; q += 4) {
+        int e = s_list[q];
+        int blk = e >> 2;
+        int addr = (blk << 6) | lane;
+        if (e & 2) {
+            old.a[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr (COULOMB) old.vc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane == 0) old.flags[blk] = 0u;
+        }
+        if (e & 1) {
+            int i, j, k;
+            block_node<DIM>(cur, blk, lane, i, j, k);
+            float4 a = cur.a[addr];
+            float vf[DIM], vc[DIM];
+            node_update<DIM, COULOMB>(ph, i, j, k, a, vf, vc);
+            cur.a[addr] = make_float4(a.x, vf[0], vf[1], DIM == 3 ? vf[DIM - 1] : 0.f);
+            if constexpr (COULOMB)
+                cur.vc[addr] = make_float4(vc[0], vc[1], DIM == 3 ? vc[DIM - 1] : 0.f, 0.f);
+        }
     }
 }
 
@@ -457,14 +478,15 @@ hipError_t launch_reset(const LaunchCfg& c, const DevGrid& g, hipStream_t st)
 hipError_t launch_grid_update_clear(const LaunchCfg& c, const DevGrid& cur, const DevGrid& old,
                                     const Phys& ph, hipStream_t st)
 {
-    dim3 grid(nblocks_total(cur)), blk(64);
+    const int nb = nblocks_total(cur);
+    dim3 grid((nb + 63) / 64), blk(256);
     bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
     if (c.dim == 2) {
-        if (coul) k_grid_update_clear<2, true><<<grid, blk, 0, st>>>(cur, old, ph);
-        else      k_grid_update_clear<2, false><<<grid, blk, 0, st>>>(cur, old, ph);
+        if (coul) k_grid_update_clear<2, true><<<grid, blk, 0, st>>>(cur, old, ph, nb);
+        else      k_grid_update_clear<2, false><<<grid, blk, 0, st>>>(cur, old, ph, nb);
     } else {
-        if (coul) k_grid_update_clear<3, true><<<grid, blk, 0, st>>>(cur, old, ph);
-        else      k_grid_update_clear<3, false><<<grid, blk, 0, st>>>(cur, old, ph);
+        if (coul) k_grid_update_clear<3, true><<<grid, blk, 0, st>>>(cur, old, ph, nb);
+        else      k_grid_update_clear<3, false><<<grid, blk, 0, st>>>(cur, old, ph, nb);
     }
     return hipGetLastError();
 }

@@ -256,14 +256,16 @@ __global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__
 }
 
 // Work items: every run of equal block id (key >> 6) in the sorted keys, split into
-// chunks of at most tile_p particles.
-__global__ void __launch_bounds__(256) k_emit_items(const uint32_t* keys, int n, int tile_p, int4* items,
-                                                   int* n_items, int max_items)
+// chunks of at most tile_p particles.  Deterministic: a run start writes its run
+// length and chunk count, an exclusive scan of the counts places the chunks, so the
+// items come out in key (spatial) order.
+__global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n, int tile_p,
+                                                    uint32_t* runlen, uint32_t* counts)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t b = keys[s] >> 6;
-    if (s > 0 && (keys[s - 1] >> 6) == b) return;
+    if (s > 0 && (keys[s - 1] >> 6) == b) { runlen[s] = 0; counts[s] = 0; return; }
     // end of the run: first index with block id > b (binary search, keys sorted)
     uint32_t next = (b + 1u) << 6;
     int lo = s + 1, hi = n;
@@ -271,13 +273,25 @@ __global__ void __launch_bounds__(256) k_emit_items(const uint32_t* keys, int n,
         int mid = (lo + hi) >> 1;
         if (keys[mid] < next) lo = mid + 1; else hi = mid;
     }
-    int count = lo - s;
-    int nch = (count + tile_p - 1) / tile_p;
-    int at = atomicAdd(n_items, nch);
-    for (int c = 0; c < nch && at + c < max_items; ++c) {
-        int first = s + c * tile_p;
-        items[at + c] = make_int4((int)b, first, min(tile_p, count - c * tile_p), 0);
-    }
+    uint32_t len = (uint32_t)(lo - s);
+    runlen[s] = len;
+    counts[s] = (len + tile_p - 1) / tile_p;
+}
+
+__global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n, int tile_p,
+                                                   const uint32_t* runlen, const uint32_t* offsets,
+                                                   int4* items, int* n_items, int max_items)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int len = (int)runlen[s];
+    int nch = (len + tile_p - 1) / tile_p;
+    int at = (int)offsets[s];
+    if (s == n - 1) *n_items = min(at + nch, max_items);
+    if (len == 0) return;
+    int b = (int)(keys[s] >> 6);
+    for (int c = 0; c < nch && at + c < max_items; ++c)
+        items[at + c] = make_int4(b, s + c * tile_p, min(tile_p, len - c * tile_p), 0);
 }
 
 __global__ void k_zero_int(int* v) { *v = 0; }
@@ -309,10 +323,16 @@ hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t*
 }
 
 hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int* n_items,
-                             int max_items, hipStream_t st)
+                             int max_items, uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
+                             hipStream_t st)
 {
     k_zero_int<<<1, 1, 0, st>>>(n_items);
-    if (n > 0) k_emit_items<<<(n + 255) / 256, 256, 0, st>>>(keys, n, tile_p, items, n_items, max_items);
+    if (n <= 0) return hipGetLastError();
+    dim3 grid((n + 255) / 256), blk(256);
+    k_item_counts<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts);
+    hipError_t e = exclusive_scan(tmp_counts, n, scan_sums, st);
+    if (e != hipSuccess) return e;
+    k_item_write<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts, items, n_items, max_items);
     return hipGetLastError();
 }
 

@@ -118,8 +118,9 @@ class OracleState:
         eye = np.zeros((d * d, n), dtype)
         for i in range(d):
             eye[i * d + i] = 1
-        return cls(np.ascontiguousarray(ps.x, dtype), np.ascontiguousarray(ps.v, dtype), np.zeros((d * d, n), dtype),
-                   eye.copy(), eye.copy(), np.ascontiguousarray(ps.mass, dtype), np.ascontiguousarray(ps.vol, dtype))
+        # always copies: the oracle steps its arrays in place
+        return cls(np.array(ps.x, dtype), np.array(ps.v, dtype), np.zeros((d * d, n), dtype),
+                   eye.copy(), eye.copy(), np.array(ps.mass, dtype), np.array(ps.vol, dtype))
 
 
 def dinv(params, dtype=np.float32):

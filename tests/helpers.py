@@ -80,15 +80,19 @@ def cell_keys(params: Params, x: np.ndarray) -> np.ndarray:
 
 
 def state_errors(a: dict, b: dict, dx: float) -> dict:
-    """Max / median absolute differences per field; x in grid cells, v and C relative
-    to the field's scale, F and Jp absolute."""
+    """Max / median absolute differences per field, each in its natural unit: x in
+    grid cells, v relative to the largest speed, C relative to the velocity-gradient
+    scale max|v|/dx (C is a cancelling sum, so max|C| is no scale when the motion is
+    nearly uniform), F and Jp absolute."""
+    vmax = max(np.abs(np.asarray(b["v"], np.float64)).max(), 1e-30)
+    scales = {"x": dx, "v": vmax, "C": vmax / dx, "F": 1.0, "Jp": 1.0}
     out = {}
     for k in ("x", "v", "C", "F", "Jp"):
         da = np.abs(np.asarray(a[k], np.float64) - np.asarray(b[k], np.float64))
-        scale = 1.0
-        if k == "x":
-            scale = dx
-        elif k in ("v", "C"):
-            scale = max(np.abs(np.asarray(b[k], np.float64)).max(), 1e-30)
-        out[k] = (float(da.max() / scale), float(np.median(da) / scale))
+        out[k] = (float(da.max() / scales[k]), float(np.median(da) / scales[k]))
     return out
+
+
+def x_ulp_cells(x: np.ndarray, dx: float) -> float:
+    """One FP32 ulp of the largest coordinate, in grid cells."""
+    return float(np.spacing(np.float32(np.abs(x).max())) / dx)

@@ -1,11 +1,16 @@
 """HIP path against the oracle, through the C-ABI (libmpm_hip.so).
 
 Tolerances (FP32 physics; the GPU sums its P2G with float atomics in an order
-that differs from the oracle's particle order, and fuses multiply-adds):
-  * one stage on identical input: relative 2e-5 of the field's largest value;
-  * N substeps of a whole scene: x within 1e-3 cells, v and C within 1e-3 of the
-    field's scale, F_E and Jp within 1e-3 (max over particles), medians 100x tighter.
-Errors against the FP64 twin of the oracle are printed next to the FP32 ones.
+that differs from the oracle's particle order, and fuses multiply-adds).  Units:
+x in grid cells, v relative to the largest speed, C relative to max|v|/dx, F_E and
+Jp absolute (tests/helpers.py:state_errors).
+  * one stage on identical input: 2e-5 (x: 2e-4 cells — the reference rebuilds the
+    position as sum_i w_i (x_i + dt v_i), whose FP32 rounding of sum_i w_i = 1 is
+    amplified by |x| / dx);
+  * 20 substeps of a whole scene, GPU against the FP32 oracle AND against the FP64
+    twin: max 1e-3, median 1e-5 (x: max 1e-3 cells, median 1e-4 cells).  The FP32
+    oracle itself is held to the same bound against FP64, so the bound is one that
+    the reference's algorithm evaluated in FP32 meets.
 """
 import numpy as np
 import pytest
@@ -68,8 +73,9 @@ def test_stage_grid_update_and_g2p(orc, case):
     ref = orc.to_solver_state(P, st)
     err = state_errors(gpu, ref, P.dx)
     _flush(f"    g2p errors {err}")
-    for k in ("x", "v", "C", "F", "Jp"):
+    for k in ("v", "C", "F", "Jp"):
         assert err[k][0] < 2e-5, (k, err[k])
+    assert err["x"][0] < 2e-4, err["x"]
     s.close()
 
 
@@ -92,8 +98,9 @@ def test_substeps_match_oracle(orc, case, fused):
     o32v64 = state_errors(orc.to_solver_state(P, o32), orc.to_solver_state(P, o64), P.dx)
     _flush(f"    gpu-vs-f32 {e32}\n    gpu-vs-f64 {e64}\n    f32-vs-f64 {o32v64}")
     for k in ("x", "v", "C", "F", "Jp"):
-        assert e32[k][0] < 1e-3, (k, e32[k])
-        assert e32[k][1] < 1e-5, (k, e32[k])
+        for e in (e32, e64, o32v64):
+            assert e[k][0] < 1e-3, (k, e[k])
+            assert e[k][1] < (1e-4 if k == "x" else 1e-5), (k, e[k])
 
 
 @pytest.mark.parametrize("case", ["snow2d_quad_sticky", "ref2d_cubic_coulomb"])
