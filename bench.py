@@ -83,7 +83,7 @@ def main() -> int:
     P = snow_params(3, 256, 5e-5)
     ps = falling_cube(seed=0, n_side=args.n_side)
     log(f"[bench] scene ready: {ps.n} particles")
-    s = GPUSolver(P, ps, Options(sort_every=args.sort_every, fused=1, tile_particles=1024))
+    s = GPUSolver(P, ps, Options(sort_every=args.sort_every, fused=1, tile_particles=512))
     init = s.get_state()
     s.Step(args.warmup)
     s.sync()

@@ -60,7 +60,7 @@ typedef struct mpm_particles {
 typedef struct mpm_options {
     int sort_every;     /* re-bin and reorder particles every k substeps of mpm_step; 0 = never */
     int fused;          /* 1: mpm_step runs the fused G2P+P2G pipeline; 0: the four reference stages */
-    int tile_particles; /* particles per work item of the tiled kernels (0 = default)            */
+    int tile_particles; /* particles per work item of the tiled kernels, 1..512 (0 = 512)         */
 } mpm_options;
 
 /* Parameters.h defaults (src/shared/Parameters.h:30-84) for dim 2 or 3. */

@@ -79,6 +79,7 @@ struct mpm_solver {
     uint32_t* radix_scratch = nullptr;
     int key_bits = 0;
     int4* items = nullptr;
+    int4* items_tmp = nullptr;
     int* n_items = nullptr;
     int max_items = 0;
     bool items_valid = false;
@@ -267,8 +268,9 @@ int do_sort(mpm_solver* s)
         // the pair of key/value buffers the sort did not end in is free now
         uint32_t* tmp_a = in_alt ? s->keys : s->keys_alt;
         uint32_t* tmp_b = in_alt ? s->vals : s->vals_alt;
-        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->n_items, s->max_items,
-                                 tmp_a, tmp_b, s->radix_scratch, s->stream);
+        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->items_tmp, s->n_items,
+                                 s->max_items, tmp_a, tmp_b, s->radix_scratch, s->dev_grid(0), s->dim,
+                                 s->stream);
     };
     int r = timed(s, MPM_KERNEL_SORT, body);
     if (r) return r;
@@ -395,7 +397,7 @@ int mpm_options_default(mpm_options* out)
     if (!out) return fail(MPM_E_INVALID, "out is NULL");
     out->sort_every = 16;
     out->fused = 1;
-    out->tile_particles = 1024;
+    out->tile_particles = 512;
     return MPM_OK;
 }
 
@@ -426,7 +428,8 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
     mpm_solver* s = new mpm_solver();
     s->params = *params;
     if (opts) s->opts = *opts; else mpm_options_default(&s->opts);
-    if (s->opts.tile_particles <= 0) s->opts.tile_particles = 1024;
+    // the tiled kernels hold at most two particles per thread of a 256-thread group
+    if (s->opts.tile_particles <= 0 || s->opts.tile_particles > 512) s->opts.tile_particles = 512;
     if (s->opts.sort_every < 0) s->opts.sort_every = 0;
     s->dim = params->dim;
     s->n = (int)init->n;
@@ -476,6 +479,7 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
         return cleanup_fail(fail(MPM_E_HIP, "binning allocation failed"));
     s->max_items = std::min(s->n, s->nblocks) + s->n / s->opts.tile_particles + 1;
     if (hipMalloc(&s->items, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&s->items_tmp, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
         hipMalloc(&s->n_items, sizeof(int)) != hipSuccess || hipMalloc(&s->d_bad, sizeof(unsigned long long)) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
     if (hipMemsetAsync(s->n_items, 0, sizeof(int), s->stream) != hipSuccess)
@@ -531,7 +535,7 @@ int mpm_destroy(mpm_solver* s)
         (void)hipFree(s->pb[g].base);
     }
     (void)hipFree(s->keys); (void)hipFree(s->vals); (void)hipFree(s->keys_alt); (void)hipFree(s->vals_alt);
-    (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
+    (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->items_tmp); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MPM_OK;

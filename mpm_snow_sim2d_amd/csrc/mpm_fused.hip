@@ -11,13 +11,22 @@
 // k_tiled<..., G2P=false> is P2G alone from the stored state (first substep).
 //
 // Work decomposition: one work item = the particles binned into one 64-node grid
-// block (at most tile_p of them, sorted by cell).  The workgroup stages the block's
-// node tile plus a halo (stencil reach + a drift margin of MG cells for particles
-// that moved since the last binning) in LDS: velocities for the gather and a zeroed
-// accumulator for the scatter.  Particle scatters are LDS float atomics; the tile is
-// flushed to HBM with one global atomic per touched node component.  A particle
-// whose stencil leaves the tile takes the direct global path, so the result never
-// depends on how stale the binning is.
+// block (at most 512 of them, sorted by cell; two per thread).  The workgroup
+// stages the block's node tile plus a halo (stencil reach + a drift margin of MG
+// cells for particles that moved since the last binning) in LDS: velocities for the
+// gather and a zeroed accumulator for the scatter.  The tile is flushed to HBM with
+// one global float atomic per touched node component.  A particle whose stencil
+// leaves the tile (or whose state is not finite) takes the direct global path, so
+// the result never depends on how stale the binning is.
+//
+// Tile accumulation is exact fixed point.  On gfx950 the LDS float atomic retires
+// about a third of a lane per clock per CU, the 64-bit integer LDS add about eight
+// (measured), so each contribution is scaled by a per-item power of two 2^e with
+// |value| * 2^e <= 2^30 (e from the item's largest per-particle bound), rounded to
+// int32 and summed in int64 LDS words: the tile sum is then exact and independent of
+// lane order, and the only rounding is the per-contribution quantisation
+// (<= 2^-31 of the item's largest contribution bound), far below FP32 summation
+// error.
 #include "mpm_kernels.hpp"
 
 namespace mpm {
@@ -47,15 +56,39 @@ __device__ inline bool in_grid(const Phys& ph, int i, int j, int k)
     return ok;
 }
 
-// Element a (0 <= a < W) of a register array, through selects (no scratch).
-template <int W>
-__device__ inline float pick(const float (&arr)[W], int a)
+// Fixed-point add of one scaled contribution into an int64 LDS accumulator.
+__device__ inline void lds_add_fixed(long long* p, float v, float scale)
 {
-    float r = arr[0];
-#pragma unroll
-    for (int s = 1; s < W; ++s) r = (a == s) ? arr[s] : r;
-    return r;
+    int q = __float2int_rn(v * scale);
+    if (q != 0) atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)(long long)q);
 }
+
+// Power-of-two scale 2^e with bound * 2^e <= 2^30.
+__device__ inline float fixed_scale(float bound)
+{
+    if (!(bound > 0.f)) return 1.f;
+    int k;
+    frexpf(bound, &k);                 // bound < 2^k
+    int e = min(max(30 - k, -126), 126);
+    return ldexpf(1.f, e);
+}
+
+__device__ inline float wave_max(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+// What the scatter needs of one particle, kept in registers between the phases.
+template <int DIM>
+struct P2GIn {
+    float xg[DIM];        // position in grid units
+    float mvp[DIM];       // m v
+    float A[DIM * DIM];   // m C - (V D^-1 dt) tau
+    float pm;
+    int mode;             // 0 none, 1 through the tile, 2 direct global atomics
+};
 
 // P2G contribution of one particle to one node: (w m, w (m v + A (x_i - x_p))).
 template <int DIM>
@@ -73,7 +106,7 @@ __device__ inline void node_contrib(float w, float pm, const float* mvp, const f
 }
 
 template <int DIM, int ORDER, bool COULOMB, bool G2P>
-__global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevGrid dst, TileWork work,
+__global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src, DevGrid dst, TileWork work,
                                                Phys ph)
 {
     using TS = TileShape<DIM>;
@@ -81,18 +114,24 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
     constexpr int W = Spline<ORDER>::W;
     constexpr int NV = G2P ? (COULOMB ? 2 * DIM : DIM) : 1;
     constexpr int TV = G2P ? TN : 1;
-    __shared__ float s_acc[DIM + 1][TN];
+    constexpr int RMAX = 2;                      // particles per thread per item
+    constexpr float REACH = ORDER == 3 ? 2.f : 1.5f;   // max |x_i - x_p| / dx per axis
+    __shared__ long long s_acc[DIM + 1][TN];
     __shared__ float s_vel[NV][TV];
     __shared__ int s_touch[TS::NB];
+    __shared__ float s_red[2][4];
 
     const int n = P.n;
     const int tid = threadIdx.x;
     const int nit = *work.n_items;
     const float fdt = ph.Dinv * ph.dt;
 
+    // Items are walked grid-stride in list order (colour-major inside each spatial
+    // chunk, see k_color_partition).  An XCD-contiguous assignment (workgroup b runs
+    // on XCD b % 8) measured slower on the falling-cube workload, so none is made.
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         int4 item = work.items[it];
-        int blk = item.x, first = item.y, cnt = item.z;
+        int blk = item.x, first = item.y, cnt = min(item.z, 256 * RMAX);
         int bc[3];
         bc[0] = blk % dst.nbx;
         int rest = blk / dst.nbx;
@@ -102,47 +141,66 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
 #pragma unroll
         for (int a = 0; a < 3; ++a) o[a] = (a < DIM) ? bc[a] * BS - 1 - MG : 0;
 
-        // ---- stage the tile
-        for (int e = tid; e < TN; e += 256) {
+        // ---- stage the tile: velocities to gather from, zeroed accumulators (all
+        // loads issued before the first LDS store, so their latencies overlap)
+        {
+            constexpr int NR = (TN + 255) / 256;
+            float4 va[NR], vb[NR];
 #pragma unroll
-            for (int c = 0; c <= DIM; ++c) s_acc[c][e] = 0.f;
-            if constexpr (G2P) {
-                int t[3];
-                tile_coords<DIM>(e, t);
-                int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];
-                float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c4 = a;
-                if (in_grid<DIM>(ph, i, j, k)) {
-                    int addr = node_addr<DIM>(src, i, j, k);
-                    a = src.a[addr];
-                    if constexpr (COULOMB) c4 = src.vc[addr];
+            for (int k = 0; k < NR; ++k) {
+                va[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                vb[k] = va[k];
+                const int e = tid + 256 * k;
+                if (G2P && e < TN) {
+                    int t[3];
+                    tile_coords<DIM>(e, t);
+                    int i = o[0] + t[0], j = o[1] + t[1], kk = o[2] + t[2];
+                    if (in_grid<DIM>(ph, i, j, kk)) {
+                        int addr = node_addr<DIM>(src, i, j, kk);
+                        va[k] = src.a[addr];
+                        if constexpr (COULOMB) vb[k] = src.vc[addr];
+                    }
                 }
-                s_vel[0][e] = a.y;
-                s_vel[1][e] = a.z;
-                if constexpr (DIM == 3) s_vel[2][e] = a.w;
-                if constexpr (COULOMB) {
-                    s_vel[DIM][e] = c4.x;
-                    s_vel[DIM + 1][e] = c4.y;
-                    if constexpr (DIM == 3) s_vel[DIM + 2][e] = c4.z;
+            }
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const int e = tid + 256 * k;
+                if (e >= TN) continue;
+#pragma unroll
+                for (int c = 0; c <= DIM; ++c) s_acc[c][e] = 0;
+                if constexpr (G2P) {
+                    s_vel[0][e] = va[k].y;
+                    s_vel[1][e] = va[k].z;
+                    if constexpr (DIM == 3) s_vel[2][e] = va[k].w;
+                    if constexpr (COULOMB) {
+                        s_vel[DIM][e] = vb[k].x;
+                        s_vel[DIM + 1][e] = vb[k].y;
+                        if constexpr (DIM == 3) s_vel[DIM + 2][e] = vb[k].z;
+                    }
                 }
             }
         }
         if (tid < TS::NB) s_touch[tid] = 0;
         __syncthreads();
 
-        for (int q = tid; q < cnt; q += 256) {
+        // ---- phase 1: per particle update (G2P) and what its scatter needs
+        P2GIn<DIM> in[RMAX];
+        float bound_m = 0.f, bound_p = 0.f;
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            in[r].mode = 0;
+            const int q = tid + 256 * r;
+            if (q >= cnt) continue;
             const int p = first + q;
-            float x[DIM], v[DIM], C[DIM * DIM], F[DIM * DIM], Jp;
-            float U[DIM * DIM], sc[DIM], tdiag[DIM];
-            float J;
+            float x[DIM], v[DIM], C[DIM * DIM], tau[DIM * DIM], Jp;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) x[d] = P.x[d * n + p];
             Jp = P.Jp[p];
             const float vol = P.vol[p];
             const float pm = P.mass[p];
-            float tau[DIM * DIM];
 
             if constexpr (G2P) {
-                // ---------------- G2P (g2p.glsl)
+                // g2p.glsl: gather, B (C = D^-1 B), position, F_E, plasticity
                 float xg[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xg[d] = x[d] * ph.inv_dx;
@@ -198,14 +256,14 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
                                 }
                             }
 #pragma unroll
-                            for (int r = 0; r < DIM; ++r) {
-                                nv[r] += vF[r] * w;
+                            for (int rr = 0; rr < DIM; ++rr) {
+                                nv[rr] += vF[rr] * w;
 #pragma unroll
-                                for (int c = 0; c < DIM; ++c) nB[r * DIM + c] += w * (vF[r] * (-dp[c]));
-                                nx[r] += (ph.dt * vC[r] + (float)idx[r] * ph.dx) * w;
+                                for (int c = 0; c < DIM; ++c) nB[rr * DIM + c] += w * (vF[rr] * (-dp[c]));
+                                nx[rr] += (ph.dt * vC[rr] + (float)idx[rr] * ph.dx) * w;
                             }
                         }
-                float M[DIM * DIM], Fo[DIM * DIM], Fh[DIM * DIM];
+                float M[DIM * DIM], Fo[DIM * DIM], Fh[DIM * DIM], F[DIM * DIM];
 #pragma unroll
                 for (int e = 0; e < DIM * DIM; ++e) {
                     C[e] = ph.Dinv * nB[e];
@@ -213,26 +271,25 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
                     Fo[e] = P.F[e * n + p];
                 }
                 matmul<DIM>(M, Fo, Fh);
-                float sig[DIM], V[DIM * DIM];
+                float U[DIM * DIM], sig[DIM], V[DIM * DIM], sc[DIM];
                 svd<DIM>(Fh, U, sig, V);
-                float ratio = 1.f;
-                J = 1.f;
+                float ratio = 1.f, J = 1.f;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    float s = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
-                    s = fmaxf(s, 1e-6f);
-                    sc[d] = s;
-                    ratio *= sig[d] / s;
-                    J *= s;
+                    float sv = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
+                    sv = fmaxf(sv, 1e-6f);
+                    sc[d] = sv;
+                    ratio *= sig[d] / sv;
+                    J *= sv;
                 }
 #pragma unroll
-                for (int r = 0; r < DIM; ++r)
+                for (int rr = 0; rr < DIM; ++rr)
 #pragma unroll
                     for (int c = 0; c < DIM; ++c) {
-                        float s = (U[r * DIM] * sc[0]) * V[c * DIM];
+                        float sacc = (U[rr * DIM] * sc[0]) * V[c * DIM];
 #pragma unroll
-                        for (int qq = 1; qq < DIM; ++qq) s += (U[r * DIM + qq] * sc[qq]) * V[c * DIM + qq];
-                        F[r * DIM + c] = s;
+                        for (int qq = 1; qq < DIM; ++qq) sacc += (U[rr * DIM + qq] * sc[qq]) * V[c * DIM + qq];
+                        F[rr * DIM + c] = sacc;
                     }
                 Jp = Jp * ratio;
 #pragma unroll
@@ -246,129 +303,144 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
                 float harden = expf(ph.xi * (1.f - Jp));
                 float tmu = 2.f * (ph.mu * harden);
                 float sI = (ph.lambda * harden) * (J - 1.f) * J;
+                float tdiag[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) tdiag[d] = tmu * (sc[d] - 1.f) * sc[d] + sI;
 #pragma unroll
-                for (int r = 0; r < DIM; ++r)
+                for (int rr = 0; rr < DIM; ++rr)
 #pragma unroll
                     for (int c = 0; c < DIM; ++c) {
-                        float s = U[r * DIM] * tdiag[0] * U[c * DIM];
+                        float sacc = U[rr * DIM] * tdiag[0] * U[c * DIM];
 #pragma unroll
-                        for (int qq = 1; qq < DIM; ++qq) s += U[r * DIM + qq] * tdiag[qq] * U[c * DIM + qq];
-                        tau[r * DIM + c] = s;
+                        for (int qq = 1; qq < DIM; ++qq) sacc += U[rr * DIM + qq] * tdiag[qq] * U[c * DIM + qq];
+                        tau[rr * DIM + c] = sacc;
                     }
             } else {
-                // ---------------- stress from the stored state (p2g.glsl:44-59)
+                // stress from the stored state (p2g.glsl:44-59)
+                float F[DIM * DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = P.v[d * n + p];
 #pragma unroll
                 for (int e = 0; e < DIM * DIM; ++e) { C[e] = P.C[e * n + p]; F[e] = P.F[e * n + p]; }
-                float sig[DIM], V[DIM * DIM];
+                float U[DIM * DIM], sig[DIM], V[DIM * DIM];
                 svd<DIM>(F, U, sig, V);
                 kirchhoff<DIM>(ph, F, U, V, det<DIM>(F), Jp, tau);
-                (void)sc; (void)tdiag; (void)J;
             }
 
             if (vol == 0.f) continue;                                 // p2g.glsl:42
-            // ---------------- P2G (p2g.glsl) of the particle into dst
             // momentum of node i: w (m v + A (x_i - x_p)), A = m C - (V D^-1 dt) tau
-            float A[DIM * DIM];
             const float vfd = vol * fdt;
+            float bp = 0.f;
 #pragma unroll
-            for (int e = 0; e < DIM * DIM; ++e) A[e] = pm * C[e] - vfd * tau[e];
-            float mvp[DIM];
+            for (int rr = 0; rr < DIM; ++rr) {
+                in[r].mvp[rr] = pm * v[rr];
+                float rowsum = 0.f;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) mvp[d] = pm * v[d];
-            float xg[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) xg[d] = x[d] * ph.inv_dx;
-            Stencil<DIM, ORDER> st;
-            st.init(xg);
-            bool tile_ok = true;
-            int rb[3] = {0, 0, 0};
-#pragma unroll
-            for (int a = 0; a < DIM; ++a) {
-                rb[a] = st.base[a] - o[a];
-                tile_ok &= rb[a] >= 0 && rb[a] + W <= T;
+                for (int c = 0; c < DIM; ++c) {
+                    float a = pm * C[rr * DIM + c] - vfd * tau[rr * DIM + c];
+                    in[r].A[rr * DIM + c] = a;
+                    rowsum += fabsf(a);
+                }
+                bp = fmaxf(bp, fabsf(in[r].mvp[rr]) + REACH * ph.dx * rowsum);
             }
-            if (tile_ok) {
-                // LDS scatter in a per-lane rotated node order: lanes that hold
-                // particles of the same cell (neighbours after binning) start at
-                // different stencil nodes, so one ds_add never sees several lanes on
-                // one address.
-                constexpr int NS = DIM == 3 ? W * W * W : W * W;
-                int r0 = (tid & 63) % NS;
-                int a2 = DIM == 3 ? r0 % W : 0;
-                int a1 = DIM == 3 ? (r0 / W) % W : r0 % W;
-                int a0 = DIM == 3 ? r0 / (W * W) : r0 / W;
+            in[r].pm = pm;
+            bool tile_ok = true;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                in[r].xg[d] = x[d] * ph.inv_dx;
+                int rb = Spline<ORDER>::base(in[r].xg[d]) - o[d];
+                tile_ok &= rb >= 0 && rb + W <= T;
+            }
+            // a bound of 0.5 * FLT_MAX keeps bound * count finite; beyond it (or NaN)
+            // the particle takes the float path so non-finite values propagate
+            bool finite = bp < 1.0e37f && fabsf(pm) < 1.0e37f;
+            if (tile_ok && finite) {
+                in[r].mode = 1;
+                bound_m = fmaxf(bound_m, fabsf(pm));
+                bound_p = fmaxf(bound_p, bp);
+            } else {
+                in[r].mode = 2;
+            }
+        }
+
+        // ---- the item's fixed-point scales
+        bound_m = wave_max(bound_m);
+        bound_p = wave_max(bound_p);
+        if ((tid & 63) == 0) { s_red[0][tid >> 6] = bound_m; s_red[1][tid >> 6] = bound_p; }
+        __syncthreads();
+        const float scale_m = fixed_scale(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])));
+        const float scale_p = fixed_scale(fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
+
+        // ---- phase 2: scatter (p2g.glsl) into the tile, or directly to HBM
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            if (in[r].mode == 0) continue;
+            int base[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                base[d] = Spline<ORDER>::base(in[r].xg[d]);
+                rb[d] = base[d] - o[d];
+            }
+            // per-lane rotated node order: lanes holding particles of one cell start at
+            // different stencil nodes, so an LDS add rarely meets another lane on its
+            // address; weights are recomputed per node from the operands Stencil::init
+            // uses (bit-identical values, no dynamically indexed arrays)
+            constexpr int NS = DIM == 3 ? W * W * W : W * W;
+            const int r0 = (tid & 63) % NS;
+            int a2 = DIM == 3 ? r0 % W : 0;
+            int a1 = DIM == 3 ? (r0 / W) % W : r0 % W;
+            int a0 = DIM == 3 ? r0 / (W * W) : r0 / W;
+            const bool via_tile = in[r].mode == 1;
 #pragma unroll 3
-                for (int sn = 0; sn < NS; ++sn) {
-                    int idx[3] = {st.base[0] + a0, st.base[1] + a1, DIM == 3 ? st.base[DIM - 1] + a2 : 0};
-                    if (in_grid<DIM>(ph, idx[0], idx[1], idx[2])) {
-                        float xi[DIM];
-                        xi[0] = -pick<W>(st.d[0], a0) * ph.dx;
-                        xi[1] = -pick<W>(st.d[1], a1) * ph.dx;
-                        float w = pick<W>(st.w[0], a0) * pick<W>(st.w[1], a1);
-                        if constexpr (DIM == 3) {
-                            xi[DIM - 1] = -pick<W>(st.d[DIM - 1], a2) * ph.dx;
-                            w = w * pick<W>(st.w[DIM - 1], a2);
-                        }
-                        float vals[DIM + 1];
-                        node_contrib<DIM>(w, pm, mvp, A, xi, vals);
+            for (int sn = 0; sn < NS; ++sn) {
+                int idx[3] = {base[0] + a0, base[1] + a1, DIM == 3 ? base[DIM - 1] + a2 : 0};
+                if (in_grid<DIM>(ph, idx[0], idx[1], idx[2])) {
+                    float xi[DIM], w = 1.f;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        float dd = in[r].xg[d] - (float)idx[d];
+                        xi[d] = -dd * ph.dx;
+                        w = d == 0 ? Spline<ORDER>::N(dd) : w * Spline<ORDER>::N(dd);
+                    }
+                    float vals[DIM + 1];
+                    node_contrib<DIM>(w, in[r].pm, in[r].mvp, in[r].A, xi, vals);
+                    if (via_tile) {
                         int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                          : (rb[1] + a1) * T + (rb[0] + a0);
+                        lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
 #pragma unroll
-                        for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc[c][e], vals[c]);
-                    }
-                    if constexpr (DIM == 3) {
-                        if (++a2 == W) { a2 = 0; if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; } }
+                        for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
                     } else {
-                        if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; }
+                        float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
+#pragma unroll
+                        for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
+                        dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
                     }
                 }
-            } else {
-                // stencil outside the tile (particle drifted since binning): direct
-                // global atomics
-#pragma unroll
-                for (int a0 = 0; a0 < W; ++a0)
-#pragma unroll
-                    for (int a1 = 0; a1 < W; ++a1)
-#pragma unroll
-                        for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
-                            int idx[3] = {st.base[0] + a0, st.base[1] + a1, DIM == 3 ? st.base[DIM - 1] + a2 : 0};
-                            if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
-                            float xi[DIM];
-                            xi[0] = -st.d[0][a0] * ph.dx;
-                            xi[1] = -st.d[1][a1] * ph.dx;
-                            float w = st.w[0][a0] * st.w[1][a1];
-                            if constexpr (DIM == 3) {
-                                xi[DIM - 1] = -st.d[DIM - 1][a2] * ph.dx;
-                                w = w * st.w[DIM - 1][a2];
-                            }
-                            float vals[DIM + 1];
-                            node_contrib<DIM>(w, pm, mvp, A, xi, vals);
-                            float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
-#pragma unroll
-                            for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
-                            dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
-                        }
+                if constexpr (DIM == 3) {
+                    if (++a2 == W) { a2 = 0; if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; } }
+                } else {
+                    if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; }
+                }
             }
         }
         __syncthreads();
 
-        // ---- flush the tile
+        // ---- flush the tile: fixed point back to FP32, one global atomic per component
+        const float inv_m = 1.f / scale_m, inv_p = 1.f / scale_p;
         for (int e = tid; e < TN; e += 256) {
-            float vals[DIM + 1];
+            long long acc[DIM + 1];
             bool any = false;
 #pragma unroll
-            for (int c = 0; c <= DIM; ++c) { vals[c] = s_acc[c][e]; any |= vals[c] != 0.f; }
+            for (int c = 0; c <= DIM; ++c) { acc[c] = s_acc[c][e]; any |= acc[c] != 0; }
             if (!any) continue;
             int t[3];
             tile_coords<DIM>(e, t);
             int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];
             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, i, j, k)]);
+            unsafeAtomicAdd(dstp, (float)((double)acc[0] * (double)inv_m));
 #pragma unroll
-            for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
+            for (int c = 1; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, (float)((double)acc[c] * (double)inv_p));
             int rbx = (i >> BlockShape<DIM>::SHIFT) - (bc[0] - 1);
             int rby = (j >> BlockShape<DIM>::SHIFT) - (bc[1] - 1);
             int rbz = DIM == 3 ? (k >> BlockShape<DIM>::SHIFT) - (bc[2] - 1) : 0;
@@ -385,10 +457,12 @@ __global__ void __launch_bounds__(256) k_tiled(DevParticles P, DevGrid src, DevG
     }
 }
 
+// Grid of the tiled kernels: a multiple of 8 (one share per XCD), at most 4096.
 static int tiled_grid(const TileWork& w)
 {
     int g = w.max_items < 4096 ? w.max_items : 4096;
-    return g > 0 ? g : 1;
+    g = (g + 7) & ~7;
+    return g > 0 ? g : 8;
 }
 
 hipError_t launch_g2p2g(const LaunchCfg& c, const DevParticles& p, const DevGrid& src, const DevGrid& dst,

@@ -102,7 +102,7 @@ class Options:
     """Solver options beyond the physics (``mpm_options``)."""
     sort_every: int = 16
     fused: int = 1
-    tile_particles: int = 1024
+    tile_particles: int = 512
 
 
 class COptions(ctypes.Structure):
