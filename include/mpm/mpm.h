@@ -117,6 +117,10 @@ int mpm_set_state(mpm_solver* s, const float* x, const float* v, const float* C,
  * (gVelCol / gVelFric).  Any pointer may be NULL. */
 int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfric);
 
+/* CPU-path scheme only: the explicit grid force f_i = sum_p grad w_ip * A_p after
+ * mpm_p2g (the force member of src/cpu/Grid.h:49), component-major. */
+int mpm_get_grid_force(mpm_solver* s, float* force);
+
 int mpm_num_particles(const mpm_solver* s, int64_t* n);
 int mpm_num_nodes(const mpm_params* p, int64_t* nodes);
 

@@ -24,6 +24,13 @@
  *   dx             grid spacing (reference: 1, grid units)
  *   mass_eps       grid nodes lighter than this are skipped by the grid update
  *                  (reference: 1e-6, shaders/2d/grid_update.glsl:90)
+ *   scheme         MPM_SCHEME_MLS: the GPU path's MLS-MPM, stress folded into the P2G
+ *                  momentum, F_E advanced with C (shaders/2d/p2g.glsl:84-86,
+ *                  g2p.glsl:74-75);
+ *                  MPM_SCHEME_APIC_FORCE: the CPU path's APIC with an explicit grid
+ *                  force f_i = sum_p V_p tau_p grad w (src/cpu/Grid.cpp:148-159,
+ *                  175-179) and F_E advanced with grad v = sum_i v_col grad w
+ *                  (src/cpu/Particles.cpp:200-214)
  */
 #ifndef MPM_PARAMS_H
 #define MPM_PARAMS_H
@@ -35,6 +42,11 @@ extern "C" {
 enum {
     MPM_BOUNDARY_COULOMB_MARGIN = 0,
     MPM_BOUNDARY_STICKY = 1
+};
+
+enum {
+    MPM_SCHEME_MLS = 0,
+    MPM_SCHEME_APIC_FORCE = 1
 };
 
 typedef struct mpm_params {
@@ -51,6 +63,7 @@ typedef struct mpm_params {
     float NU, E, RHO;
     float MU, LAMBDA;
     float mass_eps;
+    int   scheme;
 } mpm_params;
 
 #ifdef __cplusplus

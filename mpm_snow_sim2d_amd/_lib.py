@@ -25,7 +25,7 @@ EXPORTS = [
     "mpm_create", "mpm_destroy", "mpm_step",
     "mpm_p2g", "mpm_grid_update", "mpm_g2p", "mpm_reset_grid",
     "mpm_sort", "mpm_get_sort", "mpm_sync", "mpm_get_positions",
-    "mpm_get_state", "mpm_set_state", "mpm_get_grid",
+    "mpm_get_state", "mpm_set_state", "mpm_get_grid", "mpm_get_grid_force",
     "mpm_num_particles", "mpm_num_nodes",
     "mpm_set_timing", "mpm_reset_timers", "mpm_get_kernel_time",
     "mpm_check_finite", "mpm_sort_pairs_host", "mpm_last_error",
@@ -82,6 +82,7 @@ def load() -> ctypes.CDLL:
         "mpm_get_state": (i32, [vp, vp, vp, vp, vp, vp]),
         "mpm_set_state": (i32, [vp, vp, vp, vp, vp, vp]),
         "mpm_get_grid": (i32, [vp, vp, vp, vp, vp]),
+        "mpm_get_grid_force": (i32, [vp, vp]),
         "mpm_num_particles": (i32, [vp, P(i64)]),
         "mpm_num_nodes": (i32, [P(Params), P(i64)]),
         "mpm_set_timing": (i32, [vp, i32]),

@@ -70,6 +70,7 @@ struct mpm_solver {
     int nbx = 0, nby = 0, nbz = 0, nblocks = 0;
     float4* ga[2] = {nullptr, nullptr};
     float4* gvc[2] = {nullptr, nullptr};
+    float4* gf[2] = {nullptr, nullptr};
     uint32_t* gflags[2] = {nullptr, nullptr};
     int cur_grid = 0;
     int grid_state = GRID_CLEAN;
@@ -105,6 +106,7 @@ struct mpm_solver {
         DevGrid g;
         g.a = ga[which];
         g.vc = gvc[which];
+        g.f = gf[which];
         g.flags = gflags[which];
         g.nbx = nbx; g.nby = nby; g.nbz = nbz;
         return g;
@@ -181,6 +183,7 @@ Phys make_phys(const mpm_params& p)
     ph.H = p.HEIGHT;
     ph.D = p.dim == 3 ? p.DEPTH : 0;
     ph.boundary = p.boundary;
+    ph.scheme = p.scheme;
     return ph;
 }
 
@@ -192,6 +195,8 @@ int validate_params(const mpm_params* p)
         return fail(MPM_E_INVALID, "spline_order must be 2 (quadratic) or 3 (cubic)");
     if (p->boundary != MPM_BOUNDARY_COULOMB_MARGIN && p->boundary != MPM_BOUNDARY_STICKY)
         return fail(MPM_E_INVALID, "unknown boundary kind");
+    if (p->scheme != MPM_SCHEME_MLS && p->scheme != MPM_SCHEME_APIC_FORCE)
+        return fail(MPM_E_INVALID, "unknown scheme");
     if (p->WIDTH < 1 || p->HEIGHT < 1 || (p->dim == 3 && p->DEPTH < 1))
         return fail(MPM_E_INVALID, "grid size must be positive");
     if (!(p->dx > 0.f) || !(p->DELTA_TIME > 0.f)) return fail(MPM_E_INVALID, "dx and DELTA_TIME must be > 0");
@@ -459,6 +464,10 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
             if (hipMalloc(&s->gvc[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
             if (hipMemsetAsync(s->gvc[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
         }
+        if (params->scheme == MPM_SCHEME_APIC_FORCE) {
+            if (hipMalloc(&s->gf[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
+            if (hipMemsetAsync(s->gf[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
+        }
         if (hipMalloc(&s->gflags[g], (size_t)s->nblocks * sizeof(uint32_t)) != hipSuccess)
             return cleanup_fail(fail(MPM_E_HIP, "flag allocation failed"));
         if (hipMemsetAsync(s->gflags[g], 0, (size_t)s->nblocks * sizeof(uint32_t), s->stream) != hipSuccess)
@@ -531,6 +540,7 @@ int mpm_destroy(mpm_solver* s)
     for (int g = 0; g < 2; ++g) {
         (void)hipFree(s->ga[g]);
         (void)hipFree(s->gvc[g]);
+        (void)hipFree(s->gf[g]);
         (void)hipFree(s->gflags[g]);
         (void)hipFree(s->pb[g].base);
     }
@@ -546,7 +556,10 @@ int mpm_step(mpm_solver* s, int n_substeps)
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     if (n_substeps < 0) return fail(MPM_E_INVALID, "n_substeps < 0");
     if (s->n == 0 || n_substeps == 0) return MPM_OK;
-    return s->opts.fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
+    // the fused pipeline implements the MLS scheme; the CPU path's scheme runs the
+    // four stages
+    const bool fused = s->opts.fused && s->params.scheme == MPM_SCHEME_MLS;
+    return fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
 }
 
 int mpm_p2g(mpm_solver* s)
@@ -722,6 +735,31 @@ int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfr
                         }
                     }
                 }
+            }
+    return MPM_OK;
+}
+
+int mpm_get_grid_force(mpm_solver* s, float* force)
+{
+    if (!s || !force) return fail(MPM_E_INVALID, "NULL argument");
+    if (!s->gf[s->cur_grid]) return fail(MPM_E_STATE, "the solver's scheme has no grid force");
+    const mpm_params& P = s->params;
+    const int d = s->dim;
+    const size_t nodes_alloc = (size_t)s->nblocks * 64;
+    std::vector<float4> f(nodes_alloc);
+    int r;
+    if ((r = copy_to_host(s, f.data(), s->gf[s->cur_grid], nodes_alloc * sizeof(float4)))) return r;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    DevGrid g = s->dev_grid(s->cur_grid);
+    const int W1 = P.WIDTH + 1, H1 = P.HEIGHT + 1, D1 = d == 3 ? P.DEPTH + 1 : 1;
+    const size_t G = (size_t)W1 * H1 * D1;
+    for (int k = 0; k < D1; ++k)
+        for (int j = 0; j < H1; ++j)
+            for (int i = 0; i < W1; ++i) {
+                size_t id = ((size_t)k * H1 + j) * W1 + i;
+                int addr = d == 3 ? node_addr<3>(g, i, j, k) : node_addr<2>(g, i, j, 0);
+                const float comp[3] = {f[addr].x, f[addr].y, f[addr].z};
+                for (int cc = 0; cc < d; ++cc) force[cc * G + id] = comp[cc];
             }
     return MPM_OK;
 }

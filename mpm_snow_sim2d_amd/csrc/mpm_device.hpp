@@ -23,15 +23,18 @@ struct Phys {
     float mass_eps;
     int   W, H, D;                   // cells per axis; nodes 0..W
     int   boundary;
+    int   scheme;                    // MPM_SCHEME_MLS or MPM_SCHEME_APIC_FORCE
 };
 
 // Grid: block-major storage, 64 nodes per block (4x4x4 in 3D, 8x8 in 2D), one
 // float4 per node: (mass, momentum) after P2G, (mass, velocity) after the grid
 // update.  vc holds the collision velocity when it differs from the friction
-// velocity (Coulomb walls); flags marks the blocks a scatter touched.
+// velocity (Coulomb walls); f accumulates the explicit grid force of the CPU
+// path's scheme (NULL otherwise); flags marks the blocks a scatter touched.
 struct DevGrid {
     float4*   a;
     float4*   vc;
+    float4*   f;
     uint32_t* flags;
     int       nbx, nby, nbz;
 };
@@ -88,6 +91,14 @@ template <> struct Spline<3> {
         if (a < 2.0f) { float t = 2.0f - a; return t * t * t / 6.0f; }
         return 0.0f;
     }
+    // N_grad as coded (shaders/2d/common.glsl:57-69, src/cpu/Grid.h:102-116)
+    __device__ static inline float dN(float x)
+    {
+        float a = fabsf(x);
+        if (a < 1.0f) return (1.5f * a * x) - (2.0f * x);
+        if (a < 2.0f) return (-0.5f * a * x) + (2.0f * x) - ((2.0f * x) / a);
+        return 0.0f;
+    }
 };
 
 template <> struct Spline<2> {
@@ -98,6 +109,13 @@ template <> struct Spline<2> {
         float a = fabsf(x);
         if (a < 0.5f) return 0.75f - a * a;
         if (a < 1.5f) { float t = 1.5f - a; return 0.5f * t * t; }
+        return 0.0f;
+    }
+    __device__ static inline float dN(float x)
+    {
+        float a = fabsf(x);
+        if (a < 0.5f) return -2.0f * x;
+        if (a < 1.5f) return (x > 0.0f ? -1.0f : 1.0f) * (1.5f - a);
         return 0.0f;
     }
 };
@@ -250,6 +268,20 @@ __device__ inline void svd(const float* M, float* U, float* sig, float* V)
 {
     if constexpr (DIM == 2) svd2(M, U, sig, V);
     else svd3(M, U, sig, V);
+}
+
+// The CPU path's decomposition (Eigen JacobiSVD, src/cpu/Particles.cpp:31,73): singular
+// values non-negative, no sign fix.  The reference SVD with its last-column flip
+// undone gives the same products U diag(s) V^T, U V^T.
+template <int DIM>
+__device__ inline void svd_nonneg(const float* M, float* U, float* sig, float* V)
+{
+    svd<DIM>(M, U, sig, V);
+    if (sig[DIM - 1] < 0.0f) {
+        sig[DIM - 1] = -sig[DIM - 1];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) U[r * DIM + DIM - 1] = -U[r * DIM + DIM - 1];
+    }
 }
 
 // Kirchhoff stress of F with polar rotation R = U V^T (shaders/2d/p2g.glsl:44-59):

@@ -99,9 +99,11 @@ __device__ inline void collide(const Phys& P, const float* loc, const float* vel
 // momentum, gravity along y, boundary.  Returns friction velocity in `vf` and
 // collision velocity in `vc`; light nodes keep zero velocity (the reference skips
 // them, leaving reset's zeros).
-template <int DIM, bool COULOMB>
+// APIC: the CPU path's update v = mv/m + dt (g - f/m) with the explicit grid force
+// f (src/cpu/Grid.cpp:165-184) instead of v = mv/m + dt g.
+template <int DIM, bool COULOMB, bool APIC = false>
 __device__ inline void node_update(const Phys& P, int i, int j, int k, float4 a, float* vf,
-                                   float* vc)
+                                   float* vc, float4 force = make_float4(0.f, 0.f, 0.f, 0.f))
 {
     float m = a.x;
 #pragma unroll
@@ -111,7 +113,16 @@ __device__ inline void node_update(const Phys& P, int i, int j, int k, float4 a,
     vel[0] = a.y / m;
     vel[1] = a.z / m;
     if constexpr (DIM == 3) vel[2] = a.w / m;
-    vel[1] += P.gravity * P.dt;
+    if constexpr (APIC) {
+        const float fc[3] = {force.x, force.y, force.z};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            float acc = (d == 1 ? P.gravity : 0.f) + ((-fc[d]) / m);
+            vel[d] += acc * P.dt;
+        }
+    } else {
+        vel[1] += P.gravity * P.dt;
+    }
     if constexpr (COULOMB) {
         float loc[DIM];
         loc[0] = (float)i * P.dx;
@@ -151,15 +162,17 @@ __device__ inline void block_node(const DevGrid& g, int blk, int lane, int& i, i
 
 // ---------------------------------------------------------------------------
 // P2G (shaders/{2d,3d}/p2g.glsl): one thread per particle, stress from the SVD of
-// F_E, MLS scatter with hardware float atomics.
-template <int DIM, int ORDER>
+// F_E, MLS scatter with hardware float atomics.  APIC: the CPU path's rasterisation
+// (src/cpu/Grid.cpp:118-163) — APIC momentum without stress, and the explicit force
+// grad w * A_p with A_p = V tau (Particles.cpp:21-62) into the force grid.
+template <int DIM, int ORDER, bool APIC>
 __global__ void __launch_bounds__(256) k_p2g(DevParticles P, DevGrid g, Phys ph)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
     const int n = P.n;
     float vol = P.vol[p];
-    if (vol == 0.f) return;                                     // p2g.glsl:42
+    if (!APIC && vol == 0.f) return;                            // p2g.glsl:42
     float pm = P.mass[p];
     float x[DIM], v[DIM], C[DIM * DIM], F[DIM * DIM];
 #pragma unroll
@@ -169,11 +182,12 @@ __global__ void __launch_bounds__(256) k_p2g(DevParticles P, DevGrid g, Phys ph)
     float Jp = P.Jp[p];
 
     float U[DIM * DIM], sig[DIM], V[DIM * DIM], tau[DIM * DIM];
-    svd<DIM>(F, U, sig, V);
+    if constexpr (APIC) svd_nonneg<DIM>(F, U, sig, V);
+    else svd<DIM>(F, U, sig, V);
     kirchhoff<DIM>(ph, F, U, V, det<DIM>(F), Jp, tau);
     float Vt[DIM * DIM];
 #pragma unroll
-    for (int e = 0; e < DIM * DIM; ++e) Vt[e] = vol * tau[e];
+    for (int e = 0; e < DIM * DIM; ++e) Vt[e] = (APIC && vol == 0.f) ? 0.f : vol * tau[e];
 
     float xg[DIM];
 #pragma unroll
@@ -210,12 +224,34 @@ __global__ void __launch_bounds__(256) k_p2g(DevParticles P, DevGrid g, Phys ph)
                         av += C[r * DIM + c] * (-dp[c]);
                         fv += Vt[r * DIM + c] * dp[c];
                     }
-                    mv[r] = m * (v[r] + av) + w * (fv * fdt);
+                    mv[r] = APIC ? m * (v[r] + av) : m * (v[r] + av) + w * (fv * fdt);
                 }
-                float* dst = reinterpret_cast<float*>(&g.a[node_addr<DIM>(g, i, j, k)]);
+                const int addr = node_addr<DIM>(g, i, j, k);
+                float* dst = reinterpret_cast<float*>(&g.a[addr]);
                 unsafeAtomicAdd(dst + 0, m);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) unsafeAtomicAdd(dst + 1 + d, mv[d]);
+                if constexpr (APIC) {
+                    // grad w with respect to x_p (Grid.h:77-83), physical units
+                    const int aa[3] = {a0, a1, a2};
+                    float gw[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        float gc = Spline<ORDER>::dN(st.d[c][aa[c]]);
+#pragma unroll
+                        for (int q = 0; q < DIM; ++q)
+                            if (q != c) gc *= st.w[q][aa[q]];
+                        gw[c] = gc * ph.inv_dx;
+                    }
+                    float* fd = reinterpret_cast<float*>(&g.f[addr]);
+#pragma unroll
+                    for (int r = 0; r < DIM; ++r) {
+                        float f = 0.f;
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) f += gw[c] * Vt[c * DIM + r];   // glm vec * mat
+                        unsafeAtomicAdd(fd + r, f);
+                    }
+                }
             }
     // mark the (at most 2 per axis) blocks the stencil spans
 #pragma unroll
@@ -232,7 +268,7 @@ __global__ void __launch_bounds__(256) k_p2g(DevParticles P, DevGrid g, Phys ph)
 }
 
 // Grid update (shaders/{2d,3d}/grid_update.glsl) over touched blocks; one wave per block.
-template <int DIM, bool COULOMB>
+template <int DIM, bool COULOMB, bool APIC>
 __global__ void __launch_bounds__(64) k_grid_update(DevGrid g, Phys ph)
 {
     int blk = blockIdx.x;
@@ -243,7 +279,8 @@ __global__ void __launch_bounds__(64) k_grid_update(DevGrid g, Phys ph)
     int addr = (blk << 6) | lane;
     float4 a = g.a[addr];
     float vf[DIM], vc[DIM];
-    node_update<DIM, COULOMB>(ph, i, j, k, a, vf, vc);
+    if constexpr (APIC) node_update<DIM, COULOMB, true>(ph, i, j, k, a, vf, vc, g.f[addr]);
+    else node_update<DIM, COULOMB>(ph, i, j, k, a, vf, vc);
     g.a[addr] = make_float4(a.x, vf[0], vf[1], DIM == 3 ? vf[DIM - 1] : 0.f);
     if constexpr (COULOMB) g.vc[addr] = make_float4(vc[0], vc[1], DIM == 3 ? vc[DIM - 1] : 0.f, 0.f);
 }
@@ -257,12 +294,15 @@ __global__ void __launch_bounds__(64) k_reset(DevGrid g)
     int addr = (blk << 6) | threadIdx.x;
     g.a[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (g.vc) g.vc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g.f) g.f[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (threadIdx.x == 0) g.flags[blk] = 0u;
 }
 
 // G2P (shaders/{2d,3d}/g2p.glsl): gather, APIC/MLS C, position, F_E with the
-// clamped SVD, plastic determinant.
-template <int DIM, int ORDER, bool COULOMB>
+// clamped SVD, plastic determinant.  APIC: the CPU path (Particles.cpp:133-216)
+// advances F_E with grad v = sum_i v_col (x) grad w instead of C, decomposed with
+// non-negative singular values.
+template <int DIM, int ORDER, bool COULOMB, bool APIC>
 __global__ void __launch_bounds__(256) k_g2p(DevParticles P, DevGrid g, Phys ph)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,11 +314,11 @@ __global__ void __launch_bounds__(256) k_g2p(DevParticles P, DevGrid g, Phys ph)
     Stencil<DIM, ORDER> st;
     st.init(xg);
     constexpr int W = Spline<ORDER>::W;
-    float nv[DIM], nB[DIM * DIM], nx[DIM];
+    float nv[DIM], nB[DIM * DIM], nx[DIM], gv[DIM * DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) { nv[d] = 0.f; nx[d] = 0.f; }
 #pragma unroll
-    for (int e = 0; e < DIM * DIM; ++e) nB[e] = 0.f;
+    for (int e = 0; e < DIM * DIM; ++e) { nB[e] = 0.f; gv[e] = 0.f; }
 #pragma unroll
     for (int a0 = 0; a0 < W; ++a0)
 #pragma unroll
@@ -290,6 +330,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevParticles P, DevGrid g, Phys ph)
                 if (i < 0 || i > ph.W || j < 0 || j > ph.H) continue;
                 if (DIM == 3 && (k < 0 || k > ph.D)) continue;
                 int idx[3] = {i, j, k};
+                const int aa[3] = {a0, a1, a2};
                 float dp[DIM];
                 dp[0] = st.d[0][a0] * ph.dx;
                 dp[1] = st.d[1][a1] * ph.dx;
@@ -315,17 +356,33 @@ __global__ void __launch_bounds__(256) k_g2p(DevParticles P, DevGrid g, Phys ph)
                     for (int c = 0; c < DIM; ++c) nB[r * DIM + c] += w * (vF[r] * (-dp[c]));
                     nx[r] += (ph.dt * vC[r] + (float)idx[r] * ph.dx) * w;
                 }
+                if constexpr (APIC) {
+                    float gw[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        float gc = Spline<ORDER>::dN(st.d[c][aa[c]]);
+#pragma unroll
+                        for (int q = 0; q < DIM; ++q)
+                            if (q != c) gc *= st.w[q][aa[q]];
+                        gw[c] = gc * ph.inv_dx;
+                    }
+#pragma unroll
+                    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) gv[r * DIM + c] += vC[r] * gw[c];   // Particles.cpp:210
+                }
             }
     float Cn[DIM * DIM], M[DIM * DIM], Fo[DIM * DIM], Fh[DIM * DIM];
 #pragma unroll
     for (int e = 0; e < DIM * DIM; ++e) {
         Cn[e] = ph.Dinv * nB[e];
-        M[e] = ((e % (DIM + 1)) == 0 ? 1.f : 0.f) + ph.dt * Cn[e];
+        M[e] = ((e % (DIM + 1)) == 0 ? 1.f : 0.f) + ph.dt * (APIC ? gv[e] : Cn[e]);
         Fo[e] = P.F[e * n + p];
     }
     matmul<DIM>(M, Fo, Fh);
     float U[DIM * DIM], sig[DIM], V[DIM * DIM], sc[DIM];
-    svd<DIM>(Fh, U, sig, V);
+    if constexpr (APIC) svd_nonneg<DIM>(Fh, U, sig, V);
+    else svd<DIM>(Fh, U, sig, V);
     float ratio = 1.f;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -426,28 +483,32 @@ hipError_t launch_p2g(const LaunchCfg& c, const DevParticles& p, const DevGrid& 
 {
     if (p.n == 0) return hipSuccess;
     dim3 grid((p.n + 255) / 256), blk(256);
-    if (c.dim == 2) {
-        if (c.order == 3) k_p2g<2, 3><<<grid, blk, 0, st>>>(p, g, ph);
-        else              k_p2g<2, 2><<<grid, blk, 0, st>>>(p, g, ph);
-    } else {
-        if (c.order == 3) k_p2g<3, 3><<<grid, blk, 0, st>>>(p, g, ph);
-        else              k_p2g<3, 2><<<grid, blk, 0, st>>>(p, g, ph);
+    const bool apic = ph.scheme == MPM_SCHEME_APIC_FORCE;
+#define P2G_CASE(D, O)                                                                   \
+    if (c.dim == D && c.order == O) {                                                    \
+        if (apic) k_p2g<D, O, true><<<grid, blk, 0, st>>>(p, g, ph);                     \
+        else      k_p2g<D, O, false><<<grid, blk, 0, st>>>(p, g, ph);                    \
+        return hipGetLastError();                                                        \
     }
-    return hipGetLastError();
+    P2G_CASE(2, 3) P2G_CASE(2, 2) P2G_CASE(3, 3) P2G_CASE(3, 2)
+#undef P2G_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_grid_update(const LaunchCfg& c, const DevGrid& g, const Phys& ph, hipStream_t st)
 {
     dim3 grid(nblocks_total(g)), blk(64);
     bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
-    if (c.dim == 2) {
-        if (coul) k_grid_update<2, true><<<grid, blk, 0, st>>>(g, ph);
-        else      k_grid_update<2, false><<<grid, blk, 0, st>>>(g, ph);
-    } else {
-        if (coul) k_grid_update<3, true><<<grid, blk, 0, st>>>(g, ph);
-        else      k_grid_update<3, false><<<grid, blk, 0, st>>>(g, ph);
+    const bool apic = ph.scheme == MPM_SCHEME_APIC_FORCE;
+#define GU_CASE(D, CO, AP)                                                               \
+    if (c.dim == D && coul == CO && apic == AP) {                                        \
+        k_grid_update<D, CO, AP><<<grid, blk, 0, st>>>(g, ph);                           \
+        return hipGetLastError();                                                        \
     }
-    return hipGetLastError();
+    GU_CASE(2, true, false) GU_CASE(2, false, false) GU_CASE(3, true, false) GU_CASE(3, false, false)
+    GU_CASE(2, true, true) GU_CASE(2, false, true) GU_CASE(3, true, true) GU_CASE(3, false, true)
+#undef GU_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_g2p(const LaunchCfg& c, const DevParticles& p, const DevGrid& g,
@@ -456,10 +517,16 @@ hipError_t launch_g2p(const LaunchCfg& c, const DevParticles& p, const DevGrid& 
     if (p.n == 0) return hipSuccess;
     dim3 grid((p.n + 255) / 256), blk(256);
     bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+    const bool apic = ph.scheme == MPM_SCHEME_APIC_FORCE;
 #define G2P_CASE(D, O)                                                                   \
     if (c.dim == D && c.order == O) {                                                    \
-        if (coul) k_g2p<D, O, true><<<grid, blk, 0, st>>>(p, g, ph);                     \
-        else      k_g2p<D, O, false><<<grid, blk, 0, st>>>(p, g, ph);                    \
+        if (apic) {                                                                      \
+            if (coul) k_g2p<D, O, true, true><<<grid, blk, 0, st>>>(p, g, ph);           \
+            else      k_g2p<D, O, false, true><<<grid, blk, 0, st>>>(p, g, ph);          \
+        } else {                                                                         \
+            if (coul) k_g2p<D, O, true, false><<<grid, blk, 0, st>>>(p, g, ph);          \
+            else      k_g2p<D, O, false, false><<<grid, blk, 0, st>>>(p, g, ph);         \
+        }                                                                                \
         return hipGetLastError();                                                        \
     }
     G2P_CASE(2, 3) G2P_CASE(2, 2) G2P_CASE(3, 3) G2P_CASE(3, 2)

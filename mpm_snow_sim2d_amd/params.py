@@ -14,6 +14,8 @@ import numpy as np
 
 BOUNDARY_COULOMB_MARGIN = 0
 BOUNDARY_STICKY = 1
+SCHEME_MLS = 0          # the GPU path (GLSL shaders)
+SCHEME_APIC_FORCE = 1   # the CPU path (src/cpu): APIC + explicit grid force
 
 
 class Params(ctypes.Structure):
@@ -38,6 +40,7 @@ class Params(ctypes.Structure):
         ("MU", ctypes.c_float),
         ("LAMBDA", ctypes.c_float),
         ("mass_eps", ctypes.c_float),
+        ("scheme", ctypes.c_int),
     ]
 
     def derive_lame(self) -> "Params":
@@ -66,7 +69,8 @@ class Params(ctypes.Structure):
 
 
 def reference_params(dim: int = 2) -> Params:
-    """The reference's Parameters.h values (grid units, cubic spline, Coulomb walls)."""
+    """The reference's Parameters.h values (grid units, cubic spline, Coulomb walls),
+    with the GPU path's MLS scheme; ``scheme=SCHEME_APIC_FORCE`` selects the CPU path's."""
     p = Params(
         dim=dim, WIDTH=200, HEIGHT=100, DEPTH=100,            # Parameters.h:30-36
         spline_order=3, boundary=BOUNDARY_COULOMB_MARGIN,

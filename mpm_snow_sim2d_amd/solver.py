@@ -138,6 +138,12 @@ class GPUSolver:
         check(self._lib.mpm_get_grid(self._h, _ptr(m), _ptr(mom), _ptr(vc), _ptr(vf)))
         return {"mass": m, "mom": mom, "vcol": vc, "vfric": vf}
 
+    def get_grid_force(self) -> np.ndarray:
+        """CPU-path scheme: the explicit grid force after p2g(), (dim, nodes)."""
+        f = np.empty((self.dim, self.params.num_nodes), np.float32)
+        check(self._lib.mpm_get_grid_force(self._h, _ptr(f)))
+        return f
+
     def get_sort(self) -> tuple[np.ndarray, np.ndarray]:
         keys = np.empty(self.n, np.uint32)
         perm = np.empty(self.n, np.uint32)

@@ -9,6 +9,7 @@
  * once, as written.
  */
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include "mpm_oracle.h"
 
@@ -97,20 +98,36 @@ void orc_stress_f64(const mpm_params* p, const double* F, double Jp, double* tau
 
 #define DEFINE_STAGES(T, S)                                                                  \
 void orc_p2g_##S(const mpm_params* p, long n, const T* x, const T* v, const T* mass,         \
-                 const T* vol, const T* B, const T* Fe, const T* Fp, T* gmass, T* gmom)      \
+                 const T* vol, const T* B, const T* Fe, const T* Fp, T* gmass, T* gmom,      \
+                 T* gforce)                                                                  \
 {                                                                                            \
+    if (p->scheme == MPM_SCHEME_APIC_FORCE) {                                                \
+        if (p->dim == 3) p2g_apic_##S##_3(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom, gforce); \
+        else             p2g_apic_##S##_2(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom, gforce); \
+        return;                                                                              \
+    }                                                                                        \
     if (p->dim == 3) p2g_##S##_3(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom);             \
     else             p2g_##S##_2(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom);             \
 }                                                                                            \
-void orc_grid_update_##S(const mpm_params* p, const T* gmass, const T* gmom, T* gvcol,       \
-                         T* gvfric)                                                          \
+void orc_grid_update_##S(const mpm_params* p, const T* gmass, const T* gmom, const T* gforce,\
+                         T* gvcol, T* gvfric)                                                \
 {                                                                                            \
+    if (p->scheme == MPM_SCHEME_APIC_FORCE) {                                                \
+        if (p->dim == 3) grid_update_apic_##S##_3(p, gmass, gmom, gforce, gvcol, gvfric);    \
+        else             grid_update_apic_##S##_2(p, gmass, gmom, gforce, gvcol, gvfric);    \
+        return;                                                                              \
+    }                                                                                        \
     if (p->dim == 3) grid_update_##S##_3(p, gmass, gmom, gvcol, gvfric);                     \
     else             grid_update_##S##_2(p, gmass, gmom, gvcol, gvfric);                     \
 }                                                                                            \
 void orc_g2p_##S(const mpm_params* p, long n, T* x, T* v, T* B, T* Fe, T* Fp,                \
                  const T* gvcol, const T* gvfric)                                            \
 {                                                                                            \
+    if (p->scheme == MPM_SCHEME_APIC_FORCE) {                                                \
+        if (p->dim == 3) g2p_apic_##S##_3(p, n, x, v, B, Fe, Fp, gvcol, gvfric);             \
+        else             g2p_apic_##S##_2(p, n, x, v, B, Fe, Fp, gvcol, gvfric);             \
+        return;                                                                              \
+    }                                                                                        \
     if (p->dim == 3) g2p_##S##_3(p, n, x, v, B, Fe, Fp, gvcol, gvfric);                      \
     else             g2p_##S##_2(p, n, x, v, B, Fe, Fp, gvcol, gvfric);                      \
 }                                                                                            \
@@ -119,13 +136,16 @@ void orc_run_##S(const mpm_params* p, long n, int nsteps, T* x, T* v, const T* m
 {                                                                                            \
     long G = orc_num_nodes(p);                                                               \
     size_t gb = (size_t)G * sizeof(T), vb = (size_t)p->dim * G * sizeof(T);                  \
+    T* gforce = p->scheme == MPM_SCHEME_APIC_FORCE ? (T*)calloc((size_t)p->dim * G, sizeof(T)) : NULL; \
     for (int s = 0; s < nsteps; ++s) {                                                       \
-        orc_p2g_##S(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom);                          \
-        orc_grid_update_##S(p, gmass, gmom, gvcol, gvfric);                                  \
+        orc_p2g_##S(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom, gforce);                  \
+        orc_grid_update_##S(p, gmass, gmom, gforce, gvcol, gvfric);                          \
         orc_g2p_##S(p, n, x, v, B, Fe, Fp, gvcol, gvfric);                                   \
-        memset(gmass, 0, gb); memset(gmom, 0, vb);          /* reset.glsl */                 \
+        memset(gmass, 0, gb); memset(gmom, 0, vb);          /* reset.glsl, Grid::Reset */    \
         memset(gvcol, 0, vb); memset(gvfric, 0, vb);                                         \
+        if (gforce) memset(gforce, 0, vb);                                                   \
     }                                                                                        \
+    free(gforce);                                                                            \
 }
 
 DEFINE_STAGES(float, f32)

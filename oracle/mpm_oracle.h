@@ -8,6 +8,8 @@
  * Every function follows the reference's GLSL compute shaders statement by
  * statement (src/gpu/shaders/{2d,3d}/{common,p2g,grid_update,g2p,reset}.glsl),
  * in FP32 (suffix _f32) and in an FP64 twin of the same algorithm (suffix _f64).
+ * With p->scheme == MPM_SCHEME_APIC_FORCE the stages restate the CPU path instead
+ * (src/cpu/Grid.cpp, src/cpu/Particles.cpp).
  *
  * Layouts (all SoA, component-major, N particles, G grid nodes):
  *   x[d*N+p], v[d*N+p]             position, velocity
@@ -49,18 +51,19 @@ void orc_svd3_f64(const double* M, double* U, double* sig, double* V);
 void orc_stress_f32(const mpm_params* p, const float* F, float Jp, float* tau);
 void orc_stress_f64(const mpm_params* p, const double* F, double Jp, double* tau);
 
-/* Stage functions (dispatch on p->dim). */
+/* Stage functions (dispatch on p->dim and p->scheme).  gforce[d*G+id] is the CPU
+ * scheme's grid force (src/cpu/Grid.h:49); unused (may be NULL) for the MLS scheme. */
 void orc_p2g_f32(const mpm_params* p, long n, const float* x, const float* v,
                  const float* mass, const float* vol, const float* B,
-                 const float* Fe, const float* Fp, float* gmass, float* gmom);
+                 const float* Fe, const float* Fp, float* gmass, float* gmom, float* gforce);
 void orc_p2g_f64(const mpm_params* p, long n, const double* x, const double* v,
                  const double* mass, const double* vol, const double* B,
-                 const double* Fe, const double* Fp, double* gmass, double* gmom);
+                 const double* Fe, const double* Fp, double* gmass, double* gmom, double* gforce);
 
 void orc_grid_update_f32(const mpm_params* p, const float* gmass, const float* gmom,
-                         float* gvcol, float* gvfric);
+                         const float* gforce, float* gvcol, float* gvfric);
 void orc_grid_update_f64(const mpm_params* p, const double* gmass, const double* gmom,
-                         double* gvcol, double* gvfric);
+                         const double* gforce, double* gvcol, double* gvfric);
 
 void orc_g2p_f32(const mpm_params* p, long n, float* x, float* v, float* B,
                  float* Fe, float* Fp, const float* gvcol, const float* gvfric);

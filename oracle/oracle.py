@@ -61,10 +61,10 @@ def load():
         f.argtypes = [P(Params), vp, T, vp]
         f = getattr(lib, f"orc_p2g_{s}")
         f.restype = None
-        f.argtypes = [P(Params), ctypes.c_long] + [vp] * 9
+        f.argtypes = [P(Params), ctypes.c_long] + [vp] * 10
         f = getattr(lib, f"orc_grid_update_{s}")
         f.restype = None
-        f.argtypes = [P(Params)] + [vp] * 4
+        f.argtypes = [P(Params)] + [vp] * 5
         f = getattr(lib, f"orc_g2p_{s}")
         f.restype = None
         f.argtypes = [P(Params), ctypes.c_long] + [vp] * 7
@@ -154,22 +154,25 @@ def run(params, st: OracleState, nsteps: int) -> OracleState:
     return st
 
 
-def p2g(params, st: OracleState):
+def p2g(params, st: OracleState, with_force: bool = False):
+    """Grid mass and momentum (and, for the APIC-force scheme, the grid force)."""
     lib = load()
     G = lib.orc_num_nodes(ctypes.byref(params))
     gm = np.zeros(G, st.dtype)
     gmom = np.zeros((st.dim, G), st.dtype)
+    gf = np.zeros((st.dim, G), st.dtype)
     getattr(lib, f"orc_p2g_{_sfx(st.dtype)}")(ctypes.byref(params), st.n, _p(st.x), _p(st.v), _p(st.mass),
-                                              _p(st.vol), _p(st.B), _p(st.Fe), _p(st.Fp), _p(gm), _p(gmom))
-    return gm, gmom
+                                              _p(st.vol), _p(st.B), _p(st.Fe), _p(st.Fp), _p(gm), _p(gmom), _p(gf))
+    return (gm, gmom, gf) if with_force else (gm, gmom)
 
 
-def grid_update(params, gm: np.ndarray, gmom: np.ndarray):
+def grid_update(params, gm: np.ndarray, gmom: np.ndarray, gforce: np.ndarray | None = None):
     lib = load()
     vc = np.zeros_like(gmom)
     vf = np.zeros_like(gmom)
+    gf = np.zeros_like(gmom) if gforce is None else np.ascontiguousarray(gforce, gmom.dtype)
     getattr(lib, f"orc_grid_update_{_sfx(gm.dtype)}")(ctypes.byref(params), _p(gm), _p(np.ascontiguousarray(gmom)),
-                                                      _p(vc), _p(vf))
+                                                      _p(gf), _p(vc), _p(vf))
     return vc, vf
 
 

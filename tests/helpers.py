@@ -4,7 +4,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from mpm_snow_sim2d_amd.params import BOUNDARY_COULOMB_MARGIN, BOUNDARY_STICKY, Params, reference_params, snow_params
+from mpm_snow_sim2d_amd.params import (BOUNDARY_COULOMB_MARGIN, BOUNDARY_STICKY, SCHEME_APIC_FORCE, Params,
+                                       reference_params, snow_params)
 from mpm_snow_sim2d_amd.scenes import ParticleSet, jitter_lattice, reference_snowballs, _set
 
 
@@ -43,7 +44,17 @@ CASES = {
     "snow3d_quad_coulomb": (lambda: snow_params(3, 40, 5e-5, boundary=BOUNDARY_COULOMB_MARGIN),
                             lambda P, s: two_balls(3, P, s, 0.6, 3.0, 5.0,
                                                    centers=((0.3, 0.12, 0.5), (0.7, 0.14, 0.5)))),
+    # the CPU path's scheme (src/cpu): APIC + explicit grid force
+    "cpu2d_apic_coulomb": (lambda: _with_scheme(reference_params(2), SCHEME_APIC_FORCE),
+                           lambda P, s: reference_snowballs(2, seed=s, num_particles=400)),
+    "cpu3d_apic_sticky": (lambda: _with_scheme(snow_params(3, 40, 5e-5), SCHEME_APIC_FORCE),
+                          lambda P, s: two_balls(3, P, s, 0.6, 3.0, 5.0)),
 }
+
+
+def _with_scheme(P: Params, scheme: int) -> Params:
+    P.scheme = scheme
+    return P
 
 
 def _small3d_ref() -> Params:

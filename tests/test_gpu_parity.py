@@ -51,6 +51,27 @@ def test_stage_p2g(orc, case):
     s.close()
 
 
+@pytest.mark.parametrize("case", [c for c in CASES if c.startswith("cpu")])
+def test_stage_p2g_force_cpu_scheme(orc, case):
+    """The CPU path's explicit grid force (src/cpu/Grid.cpp:153) on identical input."""
+    P, ps = make_case(case)
+    _flush(f"[p2g force] {case} n={ps.n}")
+    s = _solver(P, ps, fused=0)
+    st = _oracle_state(orc, ps, np.float32)
+    # a non-trivial elastic state so the force is not zero
+    rng = np.random.default_rng(1)
+    F = st.Fe + rng.normal(scale=0.01, size=st.Fe.shape).astype(np.float32)
+    st.Fe = np.ascontiguousarray(F)
+    s.set_state(F=F)
+    s.p2g()
+    f_gpu = s.get_grid_force()
+    _, _, f_orc = orc.p2g(P, st, with_force=True)
+    scale = np.abs(f_orc).max()
+    assert scale > 0
+    assert np.abs(f_gpu - f_orc).max() <= 2e-5 * scale
+    s.close()
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_stage_grid_update_and_g2p(orc, case):
     """grid_update on the GPU's own P2G grid equals the oracle's grid update of that
@@ -60,9 +81,10 @@ def test_stage_grid_update_and_g2p(orc, case):
     s = _solver(P, ps, fused=0)
     s.p2g()
     g0 = s.get_grid()
+    force = s.get_grid_force() if P.scheme else None
     s.grid_update()
     g1 = s.get_grid()
-    vc, vf = orc.grid_update(P, g0["mass"], g0["mom"])
+    vc, vf = orc.grid_update(P, g0["mass"], g0["mom"], force)
     sc = max(np.abs(vf).max(), 1e-30)
     assert np.abs(g1["vfric"] - vf).max() <= 2e-5 * sc
     assert np.abs(g1["vcol"] - vc).max() <= 2e-5 * sc
