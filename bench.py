@@ -60,6 +60,26 @@ def cpu_baseline(params, ps, substeps: int = 2) -> dict:
                       f"full {ps.n}-particle cube on the 256^3 grid ({dt:.1f} s)"}
 
 
+def copy_calibration(gib: float = 2.0) -> None:
+    """A plain device-to-device copy of `gib` GiB (larger than the 256 MB last-level
+    cache), for calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE against bytes moved:
+    run this under `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) and compare the
+    __amd_rocclr_copyBuffer dispatch with the size printed here."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = int(gib * (1 << 30))
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_each_way": nbytes, "copies": 3}), flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +89,12 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=126)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 2 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (one process on one "
               "MI355X; the multi-GPU domain decomposition is design-only)", file=sys.stderr)
