@@ -145,6 +145,37 @@ int mpm_get_kernel_time(mpm_solver* s, int kernel, double* total_ms, int64_t* la
  * MPM_E_NONFINITE and the count of bad particles. */
 int mpm_check_finite(mpm_solver* s, int64_t* bad_particles);
 
+/* Batch of independent scenes in one solver (no reference counterpart: the
+ * reference simulates one scene per process, src/main.cpp:22-29).  Scene k has its
+ * own parameters params[k] and particles init[k]; all scenes share dim, spline
+ * order, boundary kind and the MLS scheme.  Every scene evolves as it would in a
+ * solver of its own (up to the run-to-run rounding order of floating-point grid
+ * accumulation, which a single scene has too); one mpm_step advances all of them
+ * with the same kernels.
+ * Particle arrays of the batch (mpm_get_state, mpm_set_state, mpm_get_positions)
+ * hold the scenes one after another, scene k at [first, first+count) as returned by
+ * mpm_scene_range.  The stage-level calls and the grid readbacks are single-scene
+ * only (MPM_E_STATE on a batch). */
+int mpm_batch_create(int nscenes, const mpm_params* params, const mpm_particles* init,
+                     const mpm_options* opts, mpm_solver** out);
+int mpm_num_scenes(const mpm_solver* s, int* nscenes);
+int mpm_scene_range(const mpm_solver* s, int scene, int64_t* first, int64_t* count);
+
+/* Graph replay (no reference counterpart): with enabled != 0 mpm_step records one
+ * hipGraph of 2*sort_every fused substeps (binning included) the first time it can
+ * and replays it for every whole period of later calls; the remainder runs eagerly.
+ * The replay launches the same kernels on the same buffers as eager stepping.
+ * Per-kernel timers do not see replayed substeps. */
+int mpm_set_graph(mpm_solver* s, int enabled);
+
+/* Autotuner (no reference counterpart): times trial_substeps of the fused pipeline
+ * (rounded up to whole binning periods) on fresh solvers built from the same input,
+ * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32}, and
+ * returns the fastest options and their time per substep.  nscenes = 1 tunes a
+ * single scene (params/init point to one element).  The input is not modified. */
+int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,
+                 int use_graph, mpm_options* best, double* best_ms_per_substep);
+
 /* Stable sort of n 32-bit keys on the device (the binning sort on its own, for
  * callers with their own keys).  Host arrays; out_perm[i] = index of the i-th
  * smallest key, ties in input order. key_bits <= 32. */

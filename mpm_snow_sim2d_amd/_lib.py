@@ -29,6 +29,7 @@ EXPORTS = [
     "mpm_num_particles", "mpm_num_nodes",
     "mpm_set_timing", "mpm_reset_timers", "mpm_get_kernel_time",
     "mpm_check_finite", "mpm_sort_pairs_host", "mpm_last_error",
+    "mpm_batch_create", "mpm_num_scenes", "mpm_scene_range", "mpm_set_graph", "mpm_autotune",
 ]
 
 
@@ -91,6 +92,11 @@ def load() -> ctypes.CDLL:
         "mpm_check_finite": (i32, [vp, P(i64)]),
         "mpm_sort_pairs_host": (i32, [vp, i64, i32, vp, vp]),
         "mpm_last_error": (ctypes.c_char_p, []),
+        "mpm_batch_create": (i32, [i32, P(Params), P(CParticles), P(COptions), P(vp)]),
+        "mpm_num_scenes": (i32, [vp, P(i32)]),
+        "mpm_scene_range": (i32, [vp, i32, P(i64), P(i64)]),
+        "mpm_set_graph": (i32, [vp, i32]),
+        "mpm_autotune": (i32, [i32, P(Params), P(CParticles), i32, i32, P(COptions), P(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

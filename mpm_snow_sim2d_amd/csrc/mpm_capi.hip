@@ -80,7 +80,6 @@ struct mpm_solver {
     uint32_t* radix_scratch = nullptr;
     int key_bits = 0;
     int4* items = nullptr;
-    int4* items_tmp = nullptr;
     int* n_items = nullptr;
     int max_items = 0;
     bool items_valid = false;
@@ -88,6 +87,20 @@ struct mpm_solver {
     int since_sort = 0;
 
     unsigned long long* d_bad = nullptr;
+
+    // batch of independent scenes (nscenes > 1): scene table, block -> scene,
+    // scene particle offsets; a single scene uses none of them
+    int nscenes = 1;
+    std::vector<SceneDesc> scenes_h;
+    std::vector<int64_t> p_off;
+    SceneDesc* scenes_d = nullptr;
+    int* block_scene_d = nullptr;
+    int* p_off_d = nullptr;
+
+    // hipGraph of one binning period pair (2 * sort_every substeps)
+    bool use_graph = false;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_grid = -1, graph_pb = -1, graph_period = 0;
 
     bool timing = false;
     KernelTimer timers[MPM_KERNEL_COUNT];
@@ -248,7 +261,10 @@ int do_sort(mpm_solver* s)
     const int n = s->n;
     auto body = [&]() -> hipError_t {
         DevParticles p = s->dev_particles();
-        hipError_t e = launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
+        hipError_t e = s->scenes_d
+                           ? launch_cell_keys_batch(s->dim, p.x, n, s->scenes_d, s->p_off_d, s->nscenes, s->keys,
+                                                    s->stream)
+                           : launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
         if (e != hipSuccess) return e;
         bool in_alt = false;
         e = radix_sort_pairs(s->keys, s->vals, s->keys_alt, s->vals_alt, n, s->key_bits, true,
@@ -273,9 +289,8 @@ int do_sort(mpm_solver* s)
         // the pair of key/value buffers the sort did not end in is free now
         uint32_t* tmp_a = in_alt ? s->keys : s->keys_alt;
         uint32_t* tmp_b = in_alt ? s->vals : s->vals_alt;
-        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->items_tmp, s->n_items,
-                                 s->max_items, tmp_a, tmp_b, s->radix_scratch, s->dev_grid(0), s->dim,
-                                 s->stream);
+        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->n_items, s->max_items, tmp_a,
+                                 tmp_b, s->radix_scratch, s->block_scene_d, s->stream);
     };
     int r = timed(s, MPM_KERNEL_SORT, body);
     if (r) return r;
@@ -293,6 +308,67 @@ TileWork tile_work(const mpm_solver* s)
     return w;
 }
 
+// One fused substep: binning when due, grid update + clear, fused G2P/P2G.
+int fused_substep(mpm_solver* s)
+{
+    int r;
+    if (s->opts.sort_every > 0 && s->since_sort >= s->opts.sort_every && (r = do_sort(s))) return r;
+    DevGrid cur = s->dev_grid(s->cur_grid), old = s->dev_grid(1 - s->cur_grid);
+    r = timed(s, MPM_KERNEL_GRID_UPDATE, [&] {
+        return s->scenes_d ? launch_grid_update_clear_batch(s->cfg, cur, old, s->scenes_d, s->block_scene_d,
+                                                            s->nblocks, s->stream)
+                           : launch_grid_update_clear(s->cfg, cur, old, s->ph, s->stream);
+    });
+    if (r) return r;
+    DevParticles p = s->dev_particles();
+    TileWork w = tile_work(s);
+    r = timed(s, MPM_KERNEL_G2P2G,
+              [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->scenes_d, s->stream); });
+    if (r) return r;
+    s->cur_grid = 1 - s->cur_grid;
+    s->since_sort += 1;
+    return MPM_OK;
+}
+
+void drop_graph(mpm_solver* s)
+{
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    s->graph_exec = nullptr;
+    s->graph_grid = s->graph_pb = -1;
+}
+
+// A graph replays `period` = 2 * sort_every substeps recorded from a point where the
+// next substep bins; after them the grid parity, the particle-buffer parity and the
+// binning counter are back where they started, so the same graph (same pointers)
+// applies again.  Timing events are not recorded inside it.
+int run_graph_period(mpm_solver* s, int period)
+{
+    if (s->graph_exec && (s->graph_grid != s->cur_grid || s->graph_pb != s->cur_pb || s->graph_period != period))
+        drop_graph(s);
+    if (!s->graph_exec) {
+        bool timing = s->timing;
+        s->timing = false;
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int grid0 = s->cur_grid, pb0 = s->cur_pb;
+        int r = MPM_OK;
+        for (int k = 0; k < period && r == MPM_OK; ++k) r = fused_substep(s);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &graph);
+        s->timing = timing;
+        if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        if (e != hipSuccess) return fail(MPM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) { s->graph_exec = nullptr; return fail(MPM_E_HIP, "hipGraphInstantiate failed"); }
+        if (s->cur_grid != grid0 || s->cur_pb != pb0) { drop_graph(s); return fail(MPM_E_STATE, "graph period not closed"); }
+        s->graph_grid = grid0;
+        s->graph_pb = pb0;
+        s->graph_period = period;
+    }
+    HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+    return MPM_OK;
+}
+
 int step_fused(mpm_solver* s, int nsub)
 {
     int r;
@@ -302,23 +378,22 @@ int step_fused(mpm_solver* s, int nsub)
         DevParticles p = s->dev_particles();
         DevGrid g = s->dev_grid(s->cur_grid);
         TileWork w = tile_work(s);
-        r = timed(s, MPM_KERNEL_P2G, [&] { return launch_p2g_tiled(s->cfg, p, g, w, s->ph, s->stream); });
+        r = timed(s, MPM_KERNEL_P2G,
+                  [&] { return launch_p2g_tiled(s->cfg, p, g, w, s->ph, s->scenes_d, s->stream); });
         if (r) return r;
         s->grid_state = GRID_READY;
     }
-    for (int k = 0; k < nsub; ++k) {
-        if (s->opts.sort_every > 0 && s->since_sort >= s->opts.sort_every && (r = do_sort(s))) return r;
-        DevGrid cur = s->dev_grid(s->cur_grid), old = s->dev_grid(1 - s->cur_grid);
-        r = timed(s, MPM_KERNEL_GRID_UPDATE,
-                  [&] { return launch_grid_update_clear(s->cfg, cur, old, s->ph, s->stream); });
-        if (r) return r;
-        DevParticles p = s->dev_particles();
-        TileWork w = tile_work(s);
-        r = timed(s, MPM_KERNEL_G2P2G,
-                  [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->stream); });
-        if (r) return r;
-        s->cur_grid = 1 - s->cur_grid;
-        s->since_sort += 1;
+    const int period = s->opts.sort_every > 0 ? 2 * s->opts.sort_every : 2;
+    int k = 0;
+    while (k < nsub) {
+        const bool aligned = s->opts.sort_every > 0 ? s->since_sort >= s->opts.sort_every : true;
+        if (s->use_graph && aligned && nsub - k >= period) {
+            if ((r = run_graph_period(s, period))) return r;
+            k += period;
+            continue;
+        }
+        if ((r = fused_substep(s))) return r;
+        ++k;
     }
     return MPM_OK;
 }
@@ -415,33 +490,49 @@ int mpm_num_nodes(const mpm_params* p, int64_t* nodes)
     return MPM_OK;
 }
 
-int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_options* opts,
-               mpm_solver** out)
+}  // extern "C"
+
+namespace {
+
+int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init, const mpm_options* opts,
+                mpm_solver** out)
 {
     if (!out) return fail(MPM_E_INVALID, "out is NULL");
     *out = nullptr;
-    int r = validate_params(params);
-    if (r) return r;
-    if (!init) return fail(MPM_E_INVALID, "init is NULL");
-    if (init->n < 0 || init->n > (int64_t)INT32_MAX / 16) return fail(MPM_E_INVALID, "particle count out of range");
-    if (init->n > 0 && (!init->x || !init->mass || !init->vol))
-        return fail(MPM_E_INVALID, "x, mass and vol are required");
+    if (nscenes < 1) return fail(MPM_E_INVALID, "nscenes must be >= 1");
+    if (!params || !init) return fail(MPM_E_INVALID, "params/init is NULL");
+    int r;
+    int64_t ntot = 0;
+    for (int sc = 0; sc < nscenes; ++sc) {
+        if ((r = validate_params(&params[sc]))) return r;
+        const mpm_particles& in = init[sc];
+        if (in.n < 0) return fail(MPM_E_INVALID, "particle count out of range");
+        if (in.n > 0 && (!in.x || !in.mass || !in.vol)) return fail(MPM_E_INVALID, "x, mass and vol are required");
+        ntot += in.n;
+        if (sc > 0 && (params[sc].dim != params[0].dim || params[sc].spline_order != params[0].spline_order ||
+                       params[sc].boundary != params[0].boundary || params[sc].scheme != params[0].scheme))
+            return fail(MPM_E_INVALID, "scenes of a batch must share dim, spline order, boundary and scheme");
+    }
+    if (nscenes > 1 && params[0].scheme != MPM_SCHEME_MLS)
+        return fail(MPM_E_INVALID, "a batch runs the MLS scheme");
+    if (ntot > (int64_t)INT32_MAX / 16) return fail(MPM_E_INVALID, "particle count out of range");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
         return fail(MPM_E_HIP, "no HIP device available");
 
     mpm_solver* s = new mpm_solver();
-    s->params = *params;
+    s->params = params[0];
     if (opts) s->opts = *opts; else mpm_options_default(&s->opts);
     // the tiled kernels hold at most two particles per thread of a 256-thread group
     if (s->opts.tile_particles <= 0 || s->opts.tile_particles > 512) s->opts.tile_particles = 512;
     if (s->opts.sort_every < 0) s->opts.sort_every = 0;
-    s->dim = params->dim;
-    s->n = (int)init->n;
-    s->ph = make_phys(*params);
-    s->cfg.dim = params->dim;
-    s->cfg.order = params->spline_order;
-    s->cfg.boundary = params->boundary;
+    s->dim = params[0].dim;
+    s->n = (int)ntot;
+    s->nscenes = nscenes;
+    s->ph = make_phys(params[0]);
+    s->cfg.dim = params[0].dim;
+    s->cfg.order = params[0].spline_order;
+    s->cfg.boundary = params[0].boundary;
 
     auto cleanup_fail = [&](int code) {
         mpm_destroy(s);
@@ -450,13 +541,44 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "hipStreamCreate failed"));
 
+    // grid blocks of every scene, one after another
     const int bs = s->dim == 3 ? 4 : 8;
-    s->nbx = (params->WIDTH + 1 + bs - 1) / bs;
-    s->nby = (params->HEIGHT + 1 + bs - 1) / bs;
-    s->nbz = s->dim == 3 ? (params->DEPTH + 1 + bs - 1) / bs : 1;
-    s->nblocks = s->nbx * s->nby * s->nbz;
+    std::vector<int> block_scene;
+    int64_t nb_total = 0;
+    s->p_off.assign(1, 0);
+    for (int sc = 0; sc < nscenes; ++sc) {
+        const mpm_params& P = params[sc];
+        SceneDesc sd{};
+        sd.ph = make_phys(P);
+        sd.nbx = (P.WIDTH + 1 + bs - 1) / bs;
+        sd.nby = (P.HEIGHT + 1 + bs - 1) / bs;
+        sd.nbz = s->dim == 3 ? (P.DEPTH + 1 + bs - 1) / bs : 1;
+        sd.block_off = (int)nb_total;
+        int64_t nb = (int64_t)sd.nbx * sd.nby * sd.nbz;
+        nb_total += nb;
+        if (nb_total * 64 >= ((int64_t)1 << 31)) return cleanup_fail(fail(MPM_E_INVALID, "grids too large for 32-bit node addressing"));
+        if (nscenes > 1) block_scene.insert(block_scene.end(), (size_t)nb, sc);
+        s->scenes_h.push_back(sd);
+        s->p_off.push_back(s->p_off.back() + init[sc].n);
+    }
+    s->nbx = s->scenes_h[0].nbx;
+    s->nby = s->scenes_h[0].nby;
+    s->nbz = s->scenes_h[0].nbz;
+    s->nblocks = (int)nb_total;
+    if (nscenes > 1) {
+        std::vector<int> poff32(s->p_off.begin(), s->p_off.end());
+        if (hipMalloc(&s->scenes_d, sizeof(SceneDesc) * nscenes) != hipSuccess ||
+            hipMalloc(&s->block_scene_d, sizeof(int) * block_scene.size()) != hipSuccess ||
+            hipMalloc(&s->p_off_d, sizeof(int) * poff32.size()) != hipSuccess)
+            return cleanup_fail(fail(MPM_E_HIP, "scene table allocation failed"));
+        if (hipMemcpyAsync(s->scenes_d, s->scenes_h.data(), sizeof(SceneDesc) * nscenes, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(s->block_scene_d, block_scene.data(), sizeof(int) * block_scene.size(), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(s->p_off_d, poff32.data(), sizeof(int) * poff32.size(), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            return cleanup_fail(fail(MPM_E_HIP, "scene table upload failed"));
+    }
+
     const size_t gbytes = (size_t)s->nblocks * 64 * sizeof(float4);
-    const bool coul = params->boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+    const bool coul = params[0].boundary == MPM_BOUNDARY_COULOMB_MARGIN;
     for (int g = 0; g < 2; ++g) {
         if (hipMalloc(&s->ga[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
         if (hipMemsetAsync(s->ga[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
@@ -464,7 +586,7 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
             if (hipMalloc(&s->gvc[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
             if (hipMemsetAsync(s->gvc[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
         }
-        if (params->scheme == MPM_SCHEME_APIC_FORCE) {
+        if (params[0].scheme == MPM_SCHEME_APIC_FORCE) {
             if (hipMalloc(&s->gf[g], gbytes) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "grid allocation failed"));
             if (hipMemsetAsync(s->gf[g], 0, gbytes, s->stream) != hipSuccess) return cleanup_fail(fail(MPM_E_HIP, "memset"));
         }
@@ -488,13 +610,13 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
         return cleanup_fail(fail(MPM_E_HIP, "binning allocation failed"));
     s->max_items = std::min(s->n, s->nblocks) + s->n / s->opts.tile_particles + 1;
     if (hipMalloc(&s->items, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
-        hipMalloc(&s->items_tmp, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
         hipMalloc(&s->n_items, sizeof(int)) != hipSuccess || hipMalloc(&s->d_bad, sizeof(unsigned long long)) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
     if (hipMemsetAsync(s->n_items, 0, sizeof(int), s->stream) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "memset"));
 
-    // initial state (GPUSolver.cpp:100-159 defaults for omitted fields)
+    // initial state (GPUSolver.cpp:100-159 defaults for omitted fields), scenes
+    // concatenated in order
     const int d = s->dim, n = s->n;
     if (n > 0) {
         std::vector<float> host((size_t)(d + d + d * d + d * d + 3) * n);
@@ -505,17 +627,23 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
         float* Jp = F + (size_t)d * d * n;
         float* m = Jp + n;
         float* vol = m + n;
-        std::memcpy(x, init->x, sizeof(float) * d * n);
-        if (init->v) std::memcpy(v, init->v, sizeof(float) * d * n);
-        if (init->C) std::memcpy(C, init->C, sizeof(float) * d * d * n);
-        if (init->F) std::memcpy(F, init->F, sizeof(float) * d * d * n);
-        else
-            for (int e = 0; e < d * d; ++e)
-                std::fill(F + (size_t)e * n, F + (size_t)(e + 1) * n, (e % (d + 1)) == 0 ? 1.f : 0.f);
-        if (init->Jp) std::memcpy(Jp, init->Jp, sizeof(float) * n);
-        else std::fill(Jp, Jp + n, 1.f);
-        std::memcpy(m, init->mass, sizeof(float) * n);
-        std::memcpy(vol, init->vol, sizeof(float) * n);
+        for (int sc = 0; sc < nscenes; ++sc) {
+            const mpm_particles& in = init[sc];
+            const int64_t ns = in.n, o = s->p_off[sc];
+            for (int c = 0; c < d; ++c) {
+                std::memcpy(x + (size_t)c * n + o, in.x + (size_t)c * ns, sizeof(float) * ns);
+                if (in.v) std::memcpy(v + (size_t)c * n + o, in.v + (size_t)c * ns, sizeof(float) * ns);
+            }
+            for (int e = 0; e < d * d; ++e) {
+                if (in.C) std::memcpy(C + (size_t)e * n + o, in.C + (size_t)e * ns, sizeof(float) * ns);
+                if (in.F) std::memcpy(F + (size_t)e * n + o, in.F + (size_t)e * ns, sizeof(float) * ns);
+                else std::fill(F + (size_t)e * n + o, F + (size_t)e * n + o + ns, (e % (d + 1)) == 0 ? 1.f : 0.f);
+            }
+            if (in.Jp) std::memcpy(Jp + o, in.Jp, sizeof(float) * ns);
+            else std::fill(Jp + o, Jp + o + ns, 1.f);
+            std::memcpy(m + o, in.mass, sizeof(float) * ns);
+            std::memcpy(vol + o, in.vol, sizeof(float) * ns);
+        }
         std::vector<uint32_t> pid(n);
         for (int i = 0; i < n; ++i) pid[i] = (uint32_t)i;
         ParticleBuffers& b = s->pb[0];
@@ -527,6 +655,105 @@ int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_op
         return cleanup_fail(fail(MPM_E_HIP, "sync failed"));
     }
     *out = s;
+    return MPM_OK;
+}
+
+int reject_batch(mpm_solver* s)
+{
+    if (s->nscenes > 1) return fail(MPM_E_STATE, "not available on a batch of scenes");
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_create(const mpm_params* params, const mpm_particles* init, const mpm_options* opts,
+               mpm_solver** out)
+{
+    if (!params || !init) {
+        if (out) *out = nullptr;
+        int r = validate_params(params);
+        if (r) return r;
+        return fail(MPM_E_INVALID, "init is NULL");
+    }
+    return create_impl(1, params, init, opts, out);
+}
+
+int mpm_batch_create(int nscenes, const mpm_params* params, const mpm_particles* init,
+                     const mpm_options* opts, mpm_solver** out)
+{
+    return create_impl(nscenes, params, init, opts, out);
+}
+
+int mpm_num_scenes(const mpm_solver* s, int* nscenes)
+{
+    if (!s || !nscenes) return fail(MPM_E_INVALID, "NULL argument");
+    *nscenes = s->nscenes;
+    return MPM_OK;
+}
+
+int mpm_scene_range(const mpm_solver* s, int scene, int64_t* first, int64_t* count)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (scene < 0 || scene >= s->nscenes) return fail(MPM_E_INVALID, "scene out of range");
+    if (first) *first = s->p_off[scene];
+    if (count) *count = s->p_off[scene + 1] - s->p_off[scene];
+    return MPM_OK;
+}
+
+int mpm_set_graph(mpm_solver* s, int enabled)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    s->use_graph = enabled != 0;
+    if (!s->use_graph) drop_graph(s);
+    return MPM_OK;
+}
+
+int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,
+                 int use_graph, mpm_options* best, double* best_ms_per_substep)
+{
+    if (!best) return fail(MPM_E_INVALID, "best is NULL");
+    if (trial_substeps < 1) return fail(MPM_E_INVALID, "trial_substeps must be >= 1");
+    static const int tiles[] = {128, 256, 512};
+    static const int periods[] = {4, 8, 16, 32};
+    double best_ms = -1.0;
+    mpm_options chosen{};
+    for (int tp : tiles)
+        for (int se : periods) {
+            mpm_options o{};
+            o.sort_every = se;
+            o.fused = 1;
+            o.tile_particles = tp;
+            mpm_solver* s = nullptr;
+            int r = create_impl(nscenes, params, init, &o, &s);
+            if (r) return r;
+            // whole binning periods, so that every candidate pays its binning share
+            const int period = 2 * se;
+            const int trial = ((trial_substeps + period - 1) / period) * period;
+            hipEvent_t a = nullptr, b = nullptr;
+            float ms = 0.f;
+            if (use_graph) s->use_graph = true;
+            r = mpm_step(s, period);  // warm-up: first binning, first P2G, graph capture
+            if (!r && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess))
+                r = fail(MPM_E_HIP, "hipEventCreate failed");
+            if (!r && hipEventRecord(a, s->stream) != hipSuccess) r = fail(MPM_E_HIP, "hipEventRecord failed");
+            if (!r) r = mpm_step(s, trial);
+            if (!r && (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                       hipEventElapsedTime(&ms, a, b) != hipSuccess))
+                r = fail(MPM_E_HIP, "trial timing failed");
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            mpm_destroy(s);
+            if (r) return r;
+            const double per = (double)ms / trial;
+            if (best_ms < 0.0 || per < best_ms) {
+                best_ms = per;
+                chosen = o;
+            }
+        }
+    *best = chosen;
+    if (best_ms_per_substep) *best_ms_per_substep = best_ms;
     return MPM_OK;
 }
 
@@ -545,7 +772,9 @@ int mpm_destroy(mpm_solver* s)
         (void)hipFree(s->pb[g].base);
     }
     (void)hipFree(s->keys); (void)hipFree(s->vals); (void)hipFree(s->keys_alt); (void)hipFree(s->vals_alt);
-    (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->items_tmp); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
+    (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
+    (void)hipFree(s->scenes_d); (void)hipFree(s->block_scene_d); (void)hipFree(s->p_off_d);
+    drop_graph(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MPM_OK;
@@ -559,13 +788,16 @@ int mpm_step(mpm_solver* s, int n_substeps)
     // the fused pipeline implements the MLS scheme; the CPU path's scheme runs the
     // four stages
     const bool fused = s->opts.fused && s->params.scheme == MPM_SCHEME_MLS;
+    if (!fused && s->nscenes > 1) return fail(MPM_E_STATE, "a batch of scenes runs the fused pipeline only");
     return fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
 }
 
 int mpm_p2g(mpm_solver* s)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
-    int r = prepare_stage_call(s);
+    int r = reject_batch(s);
+    if (r) return r;
+    r = prepare_stage_call(s);
     if (r) return r;
     DevParticles p = s->dev_particles();
     DevGrid g = s->dev_grid(s->cur_grid);
@@ -577,7 +809,9 @@ int mpm_p2g(mpm_solver* s)
 int mpm_grid_update(mpm_solver* s)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
-    int r = prepare_stage_call(s);
+    int r = reject_batch(s);
+    if (r) return r;
+    r = prepare_stage_call(s);
     if (r) return r;
     DevGrid g = s->dev_grid(s->cur_grid);
     r = timed(s, MPM_KERNEL_GRID_UPDATE, [&] { return launch_grid_update(s->cfg, g, s->ph, s->stream); });
@@ -588,7 +822,9 @@ int mpm_grid_update(mpm_solver* s)
 int mpm_g2p(mpm_solver* s)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
-    int r = prepare_stage_call(s);
+    int r = reject_batch(s);
+    if (r) return r;
+    r = prepare_stage_call(s);
     if (r) return r;
     DevParticles p = s->dev_particles();
     DevGrid g = s->dev_grid(s->cur_grid);
@@ -700,6 +936,7 @@ int mpm_get_positions(mpm_solver* s, float* xyz)
 int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfric)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (reject_batch(s)) return MPM_E_STATE;
     const mpm_params& P = s->params;
     const int d = s->dim;
     const size_t nodes_alloc = (size_t)s->nblocks * 64;
@@ -742,6 +979,7 @@ int mpm_get_grid(mpm_solver* s, float* mass, float* mom, float* vcol, float* vfr
 int mpm_get_grid_force(mpm_solver* s, float* force)
 {
     if (!s || !force) return fail(MPM_E_INVALID, "NULL argument");
+    if (reject_batch(s)) return MPM_E_STATE;
     if (!s->gf[s->cur_grid]) return fail(MPM_E_STATE, "the solver's scheme has no grid force");
     const mpm_params& P = s->params;
     const int d = s->dim;

@@ -39,6 +39,27 @@ struct DevGrid {
     int       nbx, nby, nbz;
 };
 
+// One scene of a batch: its parameters and where its grid blocks start in the
+// batch-wide block numbering (block-major addresses are offset by block_off*64).
+struct SceneDesc {
+    Phys ph;
+    int  block_off;
+    int  nbx, nby, nbz;
+    int  pad;
+};
+
+__host__ __device__ inline DevGrid scene_grid(const DevGrid& g, const SceneDesc& sd)
+{
+    DevGrid r;
+    const size_t off = (size_t)sd.block_off * 64;
+    r.a = g.a + off;
+    r.vc = g.vc ? g.vc + off : nullptr;
+    r.f = g.f ? g.f + off : nullptr;
+    r.flags = g.flags + sd.block_off;
+    r.nbx = sd.nbx; r.nby = sd.nby; r.nbz = sd.nbz;
+    return r;
+}
+
 struct DevParticles {
     float*       x;     // [DIM][n]
     float*       v;     // [DIM][n]

@@ -105,9 +105,9 @@ __device__ inline void node_contrib(float w, float pm, const float* mvp, const f
     }
 }
 
-template <int DIM, int ORDER, bool COULOMB, bool G2P>
-__global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src, DevGrid dst, TileWork work,
-                                               Phys ph)
+template <int DIM, int ORDER, bool COULOMB, bool G2P, bool BATCH>
+__global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_all, DevGrid dst_all,
+                                                  TileWork work, Phys ph_one, const SceneDesc* scenes)
 {
     using TS = TileShape<DIM>;
     constexpr int BS = TS::BS, MG = TS::MG, T = TS::T, TN = TS::TN;
@@ -124,14 +124,25 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src, D
     const int n = P.n;
     const int tid = threadIdx.x;
     const int nit = *work.n_items;
-    const float fdt = ph.Dinv * ph.dt;
 
-    // Items are walked grid-stride in list order (colour-major inside each spatial
-    // chunk, see k_color_partition).  An XCD-contiguous assignment (workgroup b runs
-    // on XCD b % 8) measured slower on the falling-cube workload, so none is made.
+    // Items are walked grid-stride in key (spatial) order.  Neither an XCD-contiguous
+    // assignment (workgroup b runs on XCD b % 8) nor a block-parity colour order of
+    // the items measured faster on the falling-cube workload, so neither is made.
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         int4 item = work.items[it];
-        int blk = item.x, first = item.y, cnt = min(item.z, 256 * RMAX);
+        // a batch of independent scenes: the item's scene supplies parameters and grid
+        Phys ph = ph_one;
+        DevGrid src = src_all, dst = dst_all;
+        int blk = item.x;
+        if constexpr (BATCH) {
+            const SceneDesc sd = scenes[item.w];
+            ph = sd.ph;
+            src = scene_grid(src_all, sd);
+            dst = scene_grid(dst_all, sd);
+            blk -= sd.block_off;
+        }
+        const float fdt = ph.Dinv * ph.dt;
+        int first = item.y, cnt = min(item.z, 256 * RMAX);
         int bc[3];
         bc[0] = blk % dst.nbx;
         int rest = blk / dst.nbx;
@@ -466,15 +477,21 @@ static int tiled_grid(const TileWork& w)
 }
 
 hipError_t launch_g2p2g(const LaunchCfg& c, const DevParticles& p, const DevGrid& src, const DevGrid& dst,
-                        const TileWork& w, const Phys& ph, hipStream_t st)
+                        const TileWork& w, const Phys& ph, const SceneDesc* scenes, hipStream_t st)
 {
     if (p.n == 0) return hipSuccess;
     dim3 grid(tiled_grid(w)), blk(256);
     bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+    const bool batch = scenes != nullptr;
 #define TCASE(D, O)                                                                          \
     if (c.dim == D && c.order == O) {                                                        \
-        if (coul) k_tiled<D, O, true, true><<<grid, blk, 0, st>>>(p, src, dst, w, ph);       \
-        else      k_tiled<D, O, false, true><<<grid, blk, 0, st>>>(p, src, dst, w, ph);      \
+        if (batch) {                                                                         \
+            if (coul) k_tiled<D, O, true, true, true><<<grid, blk, 0, st>>>(p, src, dst, w, ph, scenes);   \
+            else      k_tiled<D, O, false, true, true><<<grid, blk, 0, st>>>(p, src, dst, w, ph, scenes);  \
+        } else {                                                                             \
+            if (coul) k_tiled<D, O, true, true, false><<<grid, blk, 0, st>>>(p, src, dst, w, ph, scenes);  \
+            else      k_tiled<D, O, false, true, false><<<grid, blk, 0, st>>>(p, src, dst, w, ph, scenes); \
+        }                                                                                    \
         return hipGetLastError();                                                            \
     }
     TCASE(2, 3) TCASE(2, 2) TCASE(3, 3) TCASE(3, 2)
@@ -483,13 +500,15 @@ hipError_t launch_g2p2g(const LaunchCfg& c, const DevParticles& p, const DevGrid
 }
 
 hipError_t launch_p2g_tiled(const LaunchCfg& c, const DevParticles& p, const DevGrid& dst,
-                            const TileWork& w, const Phys& ph, hipStream_t st)
+                            const TileWork& w, const Phys& ph, const SceneDesc* scenes, hipStream_t st)
 {
     if (p.n == 0) return hipSuccess;
     dim3 grid(tiled_grid(w)), blk(256);
+    const bool batch = scenes != nullptr;
 #define TCASE(D, O)                                                                          \
     if (c.dim == D && c.order == O) {                                                        \
-        k_tiled<D, O, false, false><<<grid, blk, 0, st>>>(p, dst, dst, w, ph);               \
+        if (batch) k_tiled<D, O, false, false, true><<<grid, blk, 0, st>>>(p, dst, dst, w, ph, scenes);  \
+        else       k_tiled<D, O, false, false, false><<<grid, blk, 0, st>>>(p, dst, dst, w, ph, scenes); \
         return hipGetLastError();                                                            \
     }
     TCASE(2, 3) TCASE(2, 2) TCASE(3, 3) TCASE(3, 2)

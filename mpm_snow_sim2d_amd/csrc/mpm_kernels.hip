@@ -419,8 +419,9 @@ __device__ inline uint64_t lanemask_lt64()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <int DIM, bool COULOMB>
-__global__ void __launch_bounds__(256) k_grid_update_clear(DevGrid cur, DevGrid old, Phys ph, int nblocks)
+template <int DIM, bool COULOMB, bool BATCH>
+__global__ void __launch_bounds__(256) k_grid_update_clear(DevGrid cur, DevGrid old, Phys ph_one, int nblocks,
+                                                          const SceneDesc* scenes, const int* block_scene)
 {
     __shared__ int s_list[64];
     __shared__ int s_cnt;
@@ -445,8 +446,18 @@ __global__ void __launch_bounds__(256) k_grid_update_clear(DevGrid cur, DevGrid 
             if (lane == 0) old.flags[blk] = 0u;
         }
         if (e & 1) {
+            // node coordinates inside the block's own scene
+            Phys ph = ph_one;
+            DevGrid gs = cur;
+            int lb = blk;
+            if constexpr (BATCH) {
+                const SceneDesc sd = scenes[block_scene[blk]];
+                ph = sd.ph;
+                gs.nbx = sd.nbx; gs.nby = sd.nby; gs.nbz = sd.nbz;
+                lb = blk - sd.block_off;
+            }
             int i, j, k;
-            block_node<DIM>(cur, blk, lane, i, j, k);
+            block_node<DIM>(gs, lb, lane, i, j, k);
             float4 a = cur.a[addr];
             float vf[DIM], vc[DIM];
             node_update<DIM, COULOMB>(ph, i, j, k, a, vf, vc);
@@ -549,11 +560,28 @@ hipError_t launch_grid_update_clear(const LaunchCfg& c, const DevGrid& cur, cons
     dim3 grid((nb + 63) / 64), blk(256);
     bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
     if (c.dim == 2) {
-        if (coul) k_grid_update_clear<2, true><<<grid, blk, 0, st>>>(cur, old, ph, nb);
-        else      k_grid_update_clear<2, false><<<grid, blk, 0, st>>>(cur, old, ph, nb);
+        if (coul) k_grid_update_clear<2, true, false><<<grid, blk, 0, st>>>(cur, old, ph, nb, nullptr, nullptr);
+        else      k_grid_update_clear<2, false, false><<<grid, blk, 0, st>>>(cur, old, ph, nb, nullptr, nullptr);
     } else {
-        if (coul) k_grid_update_clear<3, true><<<grid, blk, 0, st>>>(cur, old, ph, nb);
-        else      k_grid_update_clear<3, false><<<grid, blk, 0, st>>>(cur, old, ph, nb);
+        if (coul) k_grid_update_clear<3, true, false><<<grid, blk, 0, st>>>(cur, old, ph, nb, nullptr, nullptr);
+        else      k_grid_update_clear<3, false, false><<<grid, blk, 0, st>>>(cur, old, ph, nb, nullptr, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_update_clear_batch(const LaunchCfg& c, const DevGrid& cur, const DevGrid& old,
+                                          const SceneDesc* scenes, const int* block_scene, int nblocks,
+                                          hipStream_t st)
+{
+    dim3 grid((nblocks + 63) / 64), blk(256);
+    bool coul = c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+    Phys none{};
+    if (c.dim == 2) {
+        if (coul) k_grid_update_clear<2, true, true><<<grid, blk, 0, st>>>(cur, old, none, nblocks, scenes, block_scene);
+        else      k_grid_update_clear<2, false, true><<<grid, blk, 0, st>>>(cur, old, none, nblocks, scenes, block_scene);
+    } else {
+        if (coul) k_grid_update_clear<3, true, true><<<grid, blk, 0, st>>>(cur, old, none, nblocks, scenes, block_scene);
+        else      k_grid_update_clear<3, false, true><<<grid, blk, 0, st>>>(cur, old, none, nblocks, scenes, block_scene);
     }
     return hipGetLastError();
 }

@@ -33,10 +33,18 @@ struct TileWork {
     const int*  n_items;
     int         max_items;
 };
+// scenes == nullptr: one scene with parameters ph; otherwise a batch whose items
+// carry their scene index (item.w) into the scene table.
 hipError_t launch_g2p2g(const LaunchCfg& c, const DevParticles& p, const DevGrid& src,
-                        const DevGrid& dst, const TileWork& w, const Phys& ph, hipStream_t st);
+                        const DevGrid& dst, const TileWork& w, const Phys& ph, const SceneDesc* scenes,
+                        hipStream_t st);
 hipError_t launch_p2g_tiled(const LaunchCfg& c, const DevParticles& p, const DevGrid& dst,
-                            const TileWork& w, const Phys& ph, hipStream_t st);
+                            const TileWork& w, const Phys& ph, const SceneDesc* scenes, hipStream_t st);
+// Batch grid pass: like launch_grid_update_clear over the batch-wide block range,
+// each block taking its scene's parameters (block_scene[b] indexes scenes).
+hipError_t launch_grid_update_clear_batch(const LaunchCfg& c, const DevGrid& cur, const DevGrid& old,
+                                          const SceneDesc* scenes, const int* block_scene, int nblocks,
+                                          hipStream_t st);
 
 // NaN/Inf scan of the particle state; count accumulated into *bad.
 hipError_t launch_check_finite(int dim, const DevParticles& p, unsigned long long* bad,

@@ -238,6 +238,34 @@ __global__ void __launch_bounds__(256) k_cell_keys(const float* x, int n, float 
     keys[p] = (uint32_t)node_addr<DIM>(g, c[0], c[1], c[2]);
 }
 
+// Batch: the slot's scene by binary search in the scene particle offsets (binning
+// never moves a particle across scenes: keys are scene-major), then the scene-local
+// key shifted by the scene's first block.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_cell_keys_batch(const float* x, int n, const SceneDesc* scenes,
+                                                        const int* p_off, int nscenes, uint32_t* keys)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int lo = 0, hi = nscenes - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (p_off[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    const SceneDesc sd = scenes[lo];
+    DevGrid g;
+    g.nbx = sd.nbx; g.nby = sd.nby; g.nbz = sd.nbz;
+    int c[3] = {0, 0, 0};
+    int lim[3] = {sd.ph.W, sd.ph.H, sd.ph.D};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        float xg = x[d * n + p] * sd.ph.inv_dx;
+        int ci = (int)floorf(xg);
+        c[d] = min(max(ci, 0), lim[d]);
+    }
+    keys[p] = (uint32_t)(sd.block_off * 64 + node_addr<DIM>(g, c[0], c[1], c[2]));
+}
+
 // Gather every particle field into sorted order: dst[s] = src[perm[s]].
 __global__ void __launch_bounds__(256) k_gather_fields(const float* __restrict__ src, float* __restrict__ dst,
                                                       const uint32_t* perm, int n, int ncomp)
@@ -256,9 +284,10 @@ __global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__
 }
 
 // Work items: every run of equal block id (key >> 6) in the sorted keys, split into
-// chunks of at most tile_p particles.  Deterministic: a run start writes its run
-// length and chunk count, an exclusive scan of the counts places the chunks, so the
-// items come out in key (spatial) order.
+// chunks of at most tile_p particles; item.w is the block's scene (0 for one
+// scene).  Deterministic: a run start writes its run length and chunk count, an
+// exclusive scan of the counts places the chunks, so the items come out in key
+// (spatial) order.
 __global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n, int tile_p,
                                                     uint32_t* runlen, uint32_t* counts)
 {
@@ -280,7 +309,8 @@ __global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n
 
 __global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n, int tile_p,
                                                    const uint32_t* runlen, const uint32_t* offsets,
-                                                   int4* items, int* n_items, int max_items)
+                                                   int4* items, int* n_items, int max_items,
+                                                   const int* block_scene)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -290,73 +320,9 @@ __global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n,
     if (s == n - 1) *n_items = min(at + nch, max_items);
     if (len == 0) return;
     int b = (int)(keys[s] >> 6);
+    int scene = block_scene ? block_scene[b] : 0;
     for (int c = 0; c < nch && at + c < max_items; ++c)
-        items[at + c] = make_int4(b, s + c * tile_p, min(tile_p, len - c * tile_p), 0);
-}
-
-// Stable partition of the work items into NCHUNK contiguous spatial chunks (one per
-// XCD of the tiled kernel's launch), each ordered by block parity colour (bx&1,
-// by&1, bz&1).  The tiled kernel hands chunk c to the workgroups of XCD c, so
-// neighbouring tiles share one L2; inside a chunk, items of one colour have tiles
-// that overlap at most in one node plane, so workgroups running at the same time
-// (consecutive items) rarely flush atomics onto the same grid nodes.  One
-// workgroup; deterministic.
-static constexpr int NCHUNK = 8;
-static constexpr int NBUCKET = NCHUNK * 8;
-
-__global__ void __launch_bounds__(256) k_color_partition(const int4* in, int4* out, const int* n_items,
-                                                        int nbx, int nby, int dim)
-{
-    __shared__ int total[NBUCKET];
-    __shared__ int base[NBUCKET];
-    __shared__ int wcnt[RS_WAVES][NBUCKET];
-    const int t = threadIdx.x, wave = t >> 6;
-    const int n = *n_items;
-    if (t < NBUCKET) total[t] = 0;
-    for (int q = t; q < RS_WAVES * NBUCKET; q += 256) wcnt[q / NBUCKET][q % NBUCKET] = 0;
-    __syncthreads();
-    auto bucket = [&](int4 it, int i) {
-        int b = it.x;
-        int bx = b % nbx, rest = b / nbx;
-        int by = dim == 3 ? rest % nby : rest;
-        int bz = dim == 3 ? rest / nby : 0;
-        int chunk = (int)(((long long)i * NCHUNK) / (n > 0 ? n : 1));
-        return chunk * 8 + ((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2));
-    };
-    for (int i = t; i < n; i += 256) atomicAdd(&total[bucket(in[i], i)], 1);
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int c = 0; c < NBUCKET; ++c) { base[c] = acc; acc += total[c]; }
-    }
-    __syncthreads();
-    for (int r = 0; r * 256 < n; ++r) {
-        int i = r * 256 + t;
-        bool valid = i < n;
-        int4 it = valid ? in[i] : make_int4(0, 0, 0, 0);
-        int c = valid ? bucket(it, i) : 0;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            uint64_t bits = __ballot((c >> b) & 1);
-            peers &= ((c >> b) & 1) ? bits : ~bits;
-        }
-        int below = __popcll(peers & lanemask_lt());
-        if (valid && below == 0) wcnt[wave][c] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = base[c] + below;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][c];
-            out[pos] = it;
-        }
-        __syncthreads();
-        if (t < NBUCKET) {
-            int add = 0;
-            for (int w = 0; w < RS_WAVES; ++w) { add += wcnt[w][t]; wcnt[w][t] = 0; }
-            base[t] += add;
-        }
-        __syncthreads();
-    }
+        items[at + c] = make_int4(b, s + c * tile_p, min(tile_p, len - c * tile_p), scene);
 }
 
 __global__ void k_zero_int(int* v) { *v = 0; }
@@ -368,6 +334,16 @@ hipError_t launch_cell_keys(int dim, const float* x, int n, const Phys& ph, cons
     dim3 grid((n + 255) / 256), blk(256);
     if (dim == 2) k_cell_keys<2><<<grid, blk, 0, st>>>(x, n, ph.inv_dx, ph.W, ph.H, ph.D, g, keys);
     else          k_cell_keys<3><<<grid, blk, 0, st>>>(x, n, ph.inv_dx, ph.W, ph.H, ph.D, g, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_keys_batch(int dim, const float* x, int n, const SceneDesc* scenes, const int* p_off,
+                                  int nscenes, uint32_t* keys, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + 255) / 256), blk(256);
+    if (dim == 2) k_cell_keys_batch<2><<<grid, blk, 0, st>>>(x, n, scenes, p_off, nscenes, keys);
+    else          k_cell_keys_batch<3><<<grid, blk, 0, st>>>(x, n, scenes, p_off, nscenes, keys);
     return hipGetLastError();
 }
 
@@ -387,9 +363,9 @@ hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int4* items_tmp,
-                             int* n_items, int max_items, uint32_t* tmp_runlen, uint32_t* tmp_counts,
-                             uint32_t* scan_sums, const DevGrid& g, int dim, hipStream_t st)
+hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int* n_items, int max_items,
+                             uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
+                             const int* block_scene, hipStream_t st)
 {
     k_zero_int<<<1, 1, 0, st>>>(n_items);
     if (n <= 0) return hipGetLastError();
@@ -397,8 +373,8 @@ hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* item
     k_item_counts<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts);
     hipError_t e = exclusive_scan(tmp_counts, n, scan_sums, st);
     if (e != hipSuccess) return e;
-    k_item_write<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts, items_tmp, n_items, max_items);
-    k_color_partition<<<1, 256, 0, st>>>(items_tmp, items, n_items, g.nbx, g.nby, dim);
+    k_item_write<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts, items, n_items, max_items,
+                                      block_scene);
     return hipGetLastError();
 }
 

@@ -227,3 +227,50 @@ def colliding_spheres(seed: int = 0, n_target: int = 16_000_000, count: int = 4,
         out = out.concat(_set(pts, v, h ** 3, rho * h ** 3))
     out.meta = {"scene": "colliding_spheres", "seed": seed, "h": h}
     return out
+
+
+def random_scene(seed: int, n_target: int, rho: float = 400.0, max_shapes: int = 3) -> ParticleSet:
+    """Random 2D snow scene in the unit square: 1..max_shapes disks (r ~ U(0.05, 0.12))
+    or boxes (half-sides ~ U(0.04, 0.10)) at seeded non-overlapping centres in
+    [0.2, 0.8]^2, each with its own velocity U(-3, 3) m/s per axis, jitter-sampled on
+    one lattice whose spacing puts about n_target particles in the union."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(1, max_shapes + 1))
+    shapes = []
+    attempts = 0
+    while len(shapes) < k and attempts < 1000:
+        attempts += 1
+        box = bool(rng.integers(0, 2))
+        c = rng.uniform(0.2, 0.8, 2)
+        half = rng.uniform(0.04, 0.10, 2) if box else np.full(2, rng.uniform(0.05, 0.12))
+        reach = float(np.hypot(*half)) if box else float(half[0])
+        if all(np.linalg.norm(c - c2) > reach + r2 + 0.01 for _, c2, _, r2 in shapes):
+            shapes.append((box, c, half, reach))
+    area = sum(4.0 * h[0] * h[1] if box else math.pi * h[0] ** 2 for box, _, h, _ in shapes)
+    h = math.sqrt(area / n_target)
+    out = ParticleSet.empty(2)
+    for box, c, half, _ in shapes:
+        if box:
+            inside = lambda p, c=c, half=half: (np.abs(p - c) <= half).all(1)
+        else:
+            inside = lambda p, c=c, r=half[0]: ((p - c) ** 2).sum(1) <= r * r
+        pts = jitter_lattice(c - half, c + half, h, rng, inside)
+        out = out.concat(_set(pts, rng.uniform(-3.0, 3.0, 2), h * h, rho * h * h))
+    out.meta = {"scene": "random_scene", "seed": seed, "h": h, "shapes": len(shapes), "n_target": n_target}
+    return out
+
+
+def batch_scenes(nscenes: int = 64, seed: int = 0, n_min: int = 5_000, n_max: int = 80_000,
+                 split: int = 20_000):
+    """Batched 2D workload: `nscenes` independent random scenes (``random_scene``)
+    with particle counts log-uniform in [n_min, n_max]; a scene under `split`
+    particles gets a 128^2 grid (dt 1e-4), a larger one 256^2 (dt 5e-5).  Returns a
+    list of (Params, ParticleSet)."""
+    from .params import snow_params
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(n_min), math.log(n_max), nscenes)).astype(int)
+    out = []
+    for i, n in enumerate(counts):
+        cells, dt = (128, 1e-4) if n < split else (256, 5e-5)
+        out.append((snow_params(2, cells, dt), random_scene(seed * 100_003 + i, int(n))))
+    return out

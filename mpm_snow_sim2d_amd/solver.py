@@ -110,6 +110,10 @@ class GPUSolver:
     def sync(self) -> None:
         check(self._lib.mpm_sync(self._h))
 
+    def set_graph(self, on: bool = True) -> None:
+        """Replay whole binning periods of ``Step`` from one recorded hipGraph."""
+        check(self._lib.mpm_set_graph(self._h, int(on)))
+
     # -- state
     def get_state(self) -> dict:
         d, n = self.dim, self.n
@@ -171,6 +175,139 @@ class GPUSolver:
         return float(ms.value), int(k.value)
 
 
+def _c_particles(ps: ParticleSet, keep: list) -> CParticles:
+    d, n = ps.dim, ps.n
+    arrs = [_f32(ps.x, (d, n)), _f32(ps.v, (d, n)), _f32(ps.mass, (n,)), _f32(ps.vol, (n,))]
+    keep.extend(arrs)
+    return CParticles(n, *[_ptr(a) for a in arrs], None, None, None)
+
+
+def _scene_arrays(scenes):
+    if not scenes:
+        raise ValueError("a batch needs at least one scene")
+    keep: list = []
+    P = (Params * len(scenes))(*[p for p, _ in scenes])
+    C = (CParticles * len(scenes))(*[_c_particles(ps, keep) for _, ps in scenes])
+    return P, C, keep
+
+
+class BatchSolver:
+    """Many independent scenes advanced together (``mpm_batch_create``).
+
+    ``scenes`` is a list of (Params, ParticleSet); state arrays hold the scenes one
+    after another in that order (``scene_range``)."""
+
+    def __init__(self, scenes, options: Options | None = None, graph: bool = False):
+        lib = _lib.load()
+        self._lib = lib
+        self.scenes = list(scenes)
+        self.dim = self.scenes[0][0].dim
+        P, C, keep = _scene_arrays(self.scenes)
+        opts = COptions.of(options or Options())
+        h = ctypes.c_void_p()
+        check(lib.mpm_batch_create(len(self.scenes), P, C, ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        del keep
+        n = ctypes.c_int64(0)
+        check(lib.mpm_num_particles(self._h, ctypes.byref(n)))
+        self.n = int(n.value)
+        if graph:
+            self.set_graph(True)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def num_scenes(self) -> int:
+        k = ctypes.c_int(0)
+        check(self._lib.mpm_num_scenes(self._h, ctypes.byref(k)))
+        return int(k.value)
+
+    def scene_range(self, scene: int) -> tuple[int, int]:
+        first = ctypes.c_int64(0)
+        count = ctypes.c_int64(0)
+        check(self._lib.mpm_scene_range(self._h, int(scene), ctypes.byref(first), ctypes.byref(count)))
+        return int(first.value), int(count.value)
+
+    def Step(self, n_substeps: int = 1) -> None:
+        check(self._lib.mpm_step(self._h, int(n_substeps)))
+
+    step = Step
+
+    def set_graph(self, on: bool = True) -> None:
+        check(self._lib.mpm_set_graph(self._h, int(on)))
+
+    def sync(self) -> None:
+        check(self._lib.mpm_sync(self._h))
+
+    def get_state(self) -> dict:
+        d, n = self.dim, self.n
+        x = np.empty((d, n), np.float32)
+        v = np.empty((d, n), np.float32)
+        C = np.empty((d * d, n), np.float32)
+        F = np.empty((d * d, n), np.float32)
+        Jp = np.empty(n, np.float32)
+        check(self._lib.mpm_get_state(self._h, _ptr(x), _ptr(v), _ptr(C), _ptr(F), _ptr(Jp)))
+        return {"x": x, "v": v, "C": C, "F": F, "Jp": Jp}
+
+    def scene_state(self, state: dict, scene: int) -> dict:
+        a, c = self.scene_range(scene)
+        return {k: v[..., a:a + c] for k, v in state.items()}
+
+    def set_state(self, x=None, v=None, C=None, F=None, Jp=None) -> None:
+        d, n = self.dim, self.n
+        arrs = [None if x is None else _f32(x, (d, n)), None if v is None else _f32(v, (d, n)),
+                None if C is None else _f32(C, (d * d, n)), None if F is None else _f32(F, (d * d, n)),
+                None if Jp is None else _f32(Jp, (n,))]
+        check(self._lib.mpm_set_state(self._h, *[_ptr(a) for a in arrs]))
+
+    def set_timing(self, on: bool = True) -> None:
+        check(self._lib.mpm_set_timing(self._h, int(on)))
+
+    def reset_timers(self) -> None:
+        check(self._lib.mpm_reset_timers(self._h))
+
+    def kernel_time(self, kernel: int) -> tuple[float, int]:
+        ms = ctypes.c_double(0.0)
+        k = ctypes.c_int64(0)
+        check(self._lib.mpm_get_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(k)))
+        return float(ms.value), int(k.value)
+
+    def check_finite(self) -> int:
+        bad = ctypes.c_int64(0)
+        code = self._lib.mpm_check_finite(self._h, ctypes.byref(bad))
+        if code not in (_lib.MPM_OK, _lib.MPM_E_NONFINITE):
+            check(code)
+        return int(bad.value)
+
+
+def autotune(scenes, trial_substeps: int = 64, graph: bool = True) -> tuple[Options, float]:
+    """``mpm_autotune``: the fastest (tile_particles, sort_every) for these scenes and
+    its time per substep in ms.  ``scenes`` is a list of (Params, ParticleSet)."""
+    lib = _lib.load()
+    P, C, keep = _scene_arrays(list(scenes))
+    best = COptions()
+    ms = ctypes.c_double(0.0)
+    check(lib.mpm_autotune(len(scenes), P, C, int(trial_substeps), int(graph), ctypes.byref(best),
+                           ctypes.byref(ms)))
+    del keep
+    return Options(best.sort_every, best.fused, best.tile_particles), float(ms.value)
+
+
 def sort_pairs(keys: np.ndarray, key_bits: int = 32) -> tuple[np.ndarray, np.ndarray]:
     """Stable device sort of uint32 keys; returns (sorted keys, permutation)."""
     lib = _lib.load()
@@ -182,4 +319,4 @@ def sort_pairs(keys: np.ndarray, key_bits: int = 32) -> tuple[np.ndarray, np.nda
     return ok, op
 
 
-__all__ = ["GPUSolver", "MPMError", "sort_pairs"]
+__all__ = ["BatchSolver", "GPUSolver", "MPMError", "autotune", "sort_pairs"]
