@@ -7,10 +7,17 @@ FP32 state.  A step is one substep (grid update + fused G2P/P2G, plus the period
 binning).  Inputs are resident in HBM when the timed region starts.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py --workload large|batch|snowballs2d [...]
+
+The other workloads print a line of the same form: `large` (16M particles in four
+colliding spheres, 512^3 grid, dt 2.5e-5), `batch` (64 independent 2D scenes of
+5k-80k particles on 128^2/256^2 grids in one solver, hipGraph replay, options from
+the autotuner unless --no-autotune) and `snowballs2d` (two 2D snowballs, ~20k
+particles, 128^2 grid, dt 1e-4).
 
 Prints one JSON line.  `roofline` prices the dominant kernel (the fused G2P+P2G
-pass) at 200 algorithmic bytes per particle (FP32 read + write of x, v, C, F_E,
-J_P) against the MI355X HBM peak; its duration comes from HIP events around each
+pass) at 200 algorithmic bytes per particle in 3D, 104 in 2D (FP32 read + write
+of x, v, C, F_E, J_P) against the MI355X HBM peak; its duration comes from HIP events around each
 launch on the solver's stream over the timed region.  `cpu_baseline` times the FP32
 CPU oracle (the same algorithm, one host thread) on 2 substeps of the same scene.
 """
@@ -30,7 +37,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "M particle-substeps/s, 3D 2M pts; learned-vs-plain speedup at ≤1% pos err"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_PARTICLE = 200.0     # 3D: (12 + 12 + 36 + 36 + 4) B read + the same written
+BYTES_PER_PARTICLE = {3: 200.0,  # 3D: (12 + 12 + 36 + 36 + 4) B read + the same written
+                      2: 104.0}  # 2D: (8 + 8 + 16 + 16 + 4) B read + the same written
 
 
 def log(msg: str) -> None:
@@ -47,7 +55,7 @@ def traffic_from_profiles() -> float | None:
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(params, ps, substeps: int = 2) -> dict:
+def cpu_baseline(params, ps, substeps: int = 2, what: str | None = None) -> dict:
     from oracle import oracle as orc
     st = orc.OracleState.from_particles(ps, np.float32)
     orc.load()
@@ -56,8 +64,9 @@ def cpu_baseline(params, ps, substeps: int = 2) -> dict:
     dt = time.perf_counter() - t0
     return {"value": ps.n * substeps / dt / 1e6, "unit": "M particle-substeps/s", "cores": 1,
             "kind": "port",
-            "sample": f"FP32 CPU oracle (GLSL-order restatement), 1 thread, {substeps} substeps of the "
-                      f"full {ps.n}-particle cube on the 256^3 grid ({dt:.1f} s)"}
+            "sample": f"FP32 CPU oracle (GLSL-order restatement), 1 thread, {substeps} substeps of "
+                      f"{what or 'the full scene'} ({ps.n} particles, grid {params.WIDTH}^{params.dim}, "
+                      f"{dt:.1f} s)"}
 
 
 def copy_calibration(gib: float = 2.0) -> None:
@@ -87,6 +96,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sort-every", type=int, default=16)
     ap.add_argument("--n-side", type=int, default=126)
+    ap.add_argument("--workload", choices=["cube", "large", "batch", "snowballs2d"], default="cube")
+    ap.add_argument("--tile", type=int, default=512)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -102,13 +114,55 @@ def main() -> int:
 
     from mpm_snow_sim2d_amd import _lib
     from mpm_snow_sim2d_amd.params import Options, snow_params
-    from mpm_snow_sim2d_amd.scenes import falling_cube
-    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.scenes import batch_scenes, colliding_spheres, falling_cube, snowball_disks
+    from mpm_snow_sim2d_amd.solver import BatchSolver, GPUSolver, autotune
 
-    P = snow_params(3, 256, 5e-5)
-    ps = falling_cube(seed=0, n_side=args.n_side)
-    log(f"[bench] scene ready: {ps.n} particles")
-    s = GPUSolver(P, ps, Options(sort_every=args.sort_every, fused=1, tile_particles=512))
+    wl = args.workload
+    graph = False
+    tuned = None
+    if wl == "cube":
+        P = snow_params(3, 256, 5e-5)
+        ps = falling_cube(seed=0, n_side=args.n_side)
+        desc = ("plain 3D snow: 0.3 m cube of 2,000,376 jittered particles dropped from 0.5 m, 256^3 grid, "
+                "dx=1/256, dt=5e-5, quadratic MLS-MPM, sticky walls")
+        grid = "256^3"
+    elif wl == "large":
+        P = snow_params(3, 512, 2.5e-5)
+        ps = colliding_spheres(seed=0)
+        desc = ("plain 3D large: four colliding spheres (r 0.08-0.15, velocities +-4 m/s), "
+                f"{ps.n} particles, 512^3 grid, dx=1/512, dt=2.5e-5, quadratic MLS-MPM, sticky walls")
+        grid = "512^3"
+    elif wl == "snowballs2d":
+        P = snow_params(2, 128, 1e-4)
+        ps = snowball_disks(seed=0)
+        desc = (f"plain 2D snow: two snowballs r=0.08 at (0.3,0.5) and (0.7,0.55), +-3 m/s, {ps.n} particles, "
+                "128^2 grid, dx=1/128, dt=1e-4, quadratic MLS-MPM, sticky walls")
+        grid = "128^2"
+    else:
+        scenes = batch_scenes(64, seed=0)
+        P = scenes[0][0]
+        graph = True
+        desc = (f"batched 2D: 64 independent random scenes (1-3 disks/boxes), "
+                f"{sum(p.n for _, p in scenes)} particles, 5k-80k per scene, 128^2 (<20k) or 256^2 grids, "
+                "hipGraph replay, quadratic MLS-MPM, sticky walls")
+        grid = "128^2/256^2 per scene"
+    dim = P.dim
+
+    if wl == "batch":
+        if args.no_autotune:
+            opts = Options(sort_every=args.sort_every, fused=1, tile_particles=args.tile)
+        else:
+            opts, tuned_ms = autotune(scenes, trial_substeps=64, graph=True)
+            tuned = {"tile_particles": opts.tile_particles, "sort_every": opts.sort_every,
+                     "ms_per_substep": round(tuned_ms, 5)}
+            log(f"[bench] autotuned {tuned}")
+        s = BatchSolver(scenes, opts)
+        n = s.n
+    else:
+        opts = Options(sort_every=args.sort_every, fused=1, tile_particles=args.tile)
+        s = GPUSolver(P, ps, opts)
+        n = ps.n
+    log(f"[bench] {wl}: {n} particles ready")
     init = s.get_state()
     s.Step(args.warmup)
     s.sync()
@@ -116,25 +170,47 @@ def main() -> int:
     s.set_state(**init)
     s.sync()
 
-    s.set_timing(True)
-    s.reset_timers()
-    s.sync()
-    t0 = time.perf_counter()
-    s.Step(args.steps)
-    s.sync()
-    wall = time.perf_counter() - t0
-    ms_k, nk = s.kernel_time(_lib.KERNEL_G2P2G)
-    ms_gu, ngu = s.kernel_time(_lib.KERNEL_GRID_UPDATE)
-    ms_sort, nsort = s.kernel_time(_lib.KERNEL_SORT)
-    ms_p2g, np2g = s.kernel_time(_lib.KERNEL_P2G)
+    def kernel_pass(steps):
+        s.set_timing(True)
+        s.reset_timers()
+        s.sync()
+        t0 = time.perf_counter()
+        s.Step(steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        t = {k: s.kernel_time(getattr(_lib, "KERNEL_" + k)) for k in ("G2P2G", "GRID_UPDATE", "SORT", "P2G")}
+        s.set_timing(False)
+        return wall, t
+
+    if graph:
+        # the timed region replays graphs, which the per-kernel events do not see; a
+        # separate eager pass of the same substeps, from the same state, times the kernels
+        s.set_graph(True)
+        s.sync()
+        t0 = time.perf_counter()
+        s.Step(args.steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        s.set_graph(False)
+        s.set_state(**init)
+        _, kt = kernel_pass(args.steps)
+    else:
+        wall, kt = kernel_pass(args.steps)
+    (ms_k, nk), (ms_gu, ngu), (ms_sort, nsort), (ms_p2g, np2g) = (kt["G2P2G"], kt["GRID_UPDATE"], kt["SORT"],
+                                                                  kt["P2G"])
     bad = s.check_finite()
     s.close()
     log(f"[bench] timed {args.steps} substeps in {wall:.3f} s; g2p2g {nk} launches {ms_k:.2f} ms; "
         f"grid {ngu} launches {ms_gu:.2f} ms; sort {nsort} {ms_sort:.2f} ms; p2g {np2g} {ms_p2g:.2f} ms; bad={bad}")
 
-    value = ps.n * args.steps / wall / 1e6
+    value = n * args.steps / wall / 1e6
     avg_k = ms_k / max(nk, 1) / 1e3
-    achieved = ps.n * BYTES_PER_PARTICLE / avg_k / 1e9 if nk else None
+    achieved = n * BYTES_PER_PARTICLE[dim] / avg_k / 1e9 if nk else None
+    kname = f"k_tiled<{dim},2,sticky,G2P{',batch' if wl == 'batch' else ''}> (fused G2P+P2G)"
+    config = {"workload": desc, "particles": n, "grid": grid, "sort_every": opts.sort_every,
+              "tile_particles": opts.tile_particles, "hipgraph": graph, "nonfinite_particles": bad}
+    if tuned:
+        config["autotuned"] = tuned
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -148,14 +224,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain 3D snow: 0.3 m cube of 2,000,376 jittered particles dropped from "
-                               "0.5 m, 256^3 grid, dx=1/256, dt=5e-5, quadratic MLS-MPM, sticky walls",
-                   "particles": ps.n, "grid": "256^3", "sort_every": args.sort_every,
-                   "nonfinite_particles": bad},
-        "roofline": {"bound": "hbm", "kernel": "k_tiled<3,2,sticky,G2P> (fused G2P+P2G)",
+        "config": config,
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-                     "traffic": traffic_from_profiles(),
+                     "traffic": traffic_from_profiles() if wl == "cube" else None,
                      "avg_launch_us": round(avg_k * 1e6, 3),
                      "grid_update_avg_us": round(ms_gu / max(ngu, 1) * 1e3, 3),
                      "sort_avg_us": round(ms_sort / max(nsort, 1) * 1e3, 3)},
@@ -163,7 +236,11 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(P, ps, args.cpu_substeps)
+        if wl == "batch":
+            big = max(scenes, key=lambda t: t[1].n)
+            out["cpu_baseline"] = cpu_baseline(big[0], big[1], args.cpu_substeps, "the largest scene of the batch")
+        else:
+            out["cpu_baseline"] = cpu_baseline(P, ps, args.cpu_substeps)
     print(json.dumps(out), flush=True)
     return 0
 
