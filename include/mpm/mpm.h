@@ -176,6 +176,61 @@ int mpm_set_graph(mpm_solver* s, int enabled);
 int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,
                  int use_graph, mpm_options* best, double* best_ms_per_substep);
 
+/* ---------------------------------------------------------------------------
+ * Learned grid surrogate (no reference counterpart).  A 4-level BF16 U-Net
+ * (32-64-128-128 channels, 3x3 convolutions, GELU; FP32 accumulation on the matrix
+ * cores) over the grid nodes of a 2D quadratic-spline scene of height x width cells
+ * predicts the grid velocity k substeps ahead from node features [mass, velocity,
+ * mean J_P, mean log singular value of F_E].  Its training set is captured from
+ * solvers into device-resident slots, training runs on the device (forward,
+ * backward, Adam), and mpm_surrogate_advance moves a single-scene solver k
+ * substeps with the predicted velocity or, when the residual test fails, with k
+ * plain substeps.
+ */
+typedef struct mpm_surrogate mpm_surrogate;
+
+enum { MPM_ADVANCE_PLAIN = 0, MPM_ADVANCE_NETWORK = 1, MPM_ADVANCE_BALLISTIC = 2 };
+
+/* height, width: image of grid nodes (multiples of 8; a scene must have WIDTH =
+ * width and HEIGHT = height cells); max_batch: images per training step; capacity:
+ * training-set slots; seed: weight initialisation. */
+int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uint64_t seed,
+                         mpm_surrogate** out);
+int mpm_surrogate_destroy(mpm_surrogate* g);
+/* Parameters, FP32: per convolution layer (e0 e1 e2 e3 d2 d1 d0) W[9][cout][cin]
+ * (tap = 3*(dy+1) + (dx+1)) then b[cout]; then the head wh[2][32], bh[2]. */
+int mpm_surrogate_num_params(const mpm_surrogate* g, int64_t* n);
+int mpm_surrogate_get_params(mpm_surrogate* g, float* host);
+int mpm_surrogate_set_params(mpm_surrogate* g, const float* host);
+/* Velocity scale of the network output (m/s; default 0.1). */
+int mpm_surrogate_set_scale(mpm_surrogate* g, float d_ref);
+/* Network on host data: input [nb][h][w][32] BF16 bit patterns, wm [nb][h*w]
+ * projection weights (summing to one per image), target [nb][h*w][2].  forward
+ * returns the projected output [nb][h*w][2]; gradient returns the loss
+ * (1/nb) sum_i wm_i |out_i - target_i|^2 and, when grad != NULL, dloss/dparams. */
+int mpm_surrogate_forward(mpm_surrogate* g, int nb, const uint16_t* input, const float* wm, float* out);
+int mpm_surrogate_gradient(mpm_surrogate* g, int nb, const uint16_t* input, const float* wm,
+                           const float* target, float* loss, float* grad);
+/* Capture scene `scene` of solver s into training slot `slot`: what = 0 the node
+ * features now (and the particle positions), what = 1 the target velocity now,
+ * at the positions of the last what = 0 capture; k_substeps = the horizon. */
+int mpm_surrogate_record(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int what, int k_substeps);
+/* Raw sample of a slot: host [7][h*w] (mass, vx, vy, J_P, log sigma, target vx, vy)
+ * and scalars [6] (particle mass, grid mass, wall-band mass, horizon T, gravity,
+ * m_ref); either may be NULL.  slot == capacity reads the advance call's sample. */
+int mpm_surrogate_get_sample(mpm_surrogate* g, int slot, float* host, float* scalars);
+/* One Adam step (lr > 0) on the batch of slots, or the loss only (lr == 0). */
+int mpm_surrogate_train(mpm_surrogate* g, const int* slots, int nb, float lr, float* loss);
+/* Loss of the zero increment (ballistic grid velocity) on the same samples. */
+int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, float* loss);
+/* Advance a single-scene solver by k_substeps: mode MPM_ADVANCE_NETWORK or
+ * MPM_ADVANCE_BALLISTIC (zero increment) when the residual (share of the mass in
+ * the wall band or outside the image) is <= tol, else k plain substeps;
+ * MPM_ADVANCE_PLAIN always steps.  *used: the mode that ran; *residual: the test's
+ * value. */
+int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,
+                          float* residual);
+
 /* Stable sort of n 32-bit keys on the device (the binning sort on its own, for
  * callers with their own keys).  Host arrays; out_perm[i] = index of the i-th
  * smallest key, ties in input order. key_bits <= 32. */

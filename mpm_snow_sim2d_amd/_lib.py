@@ -17,6 +17,8 @@ MPM_E_HIP = -2
 MPM_E_STATE = -3
 MPM_E_NONFINITE = -4
 
+ADVANCE_PLAIN, ADVANCE_NETWORK, ADVANCE_BALLISTIC = 0, 1, 2
+
 KERNEL_P2G, KERNEL_GRID_UPDATE, KERNEL_G2P, KERNEL_RESET, KERNEL_G2P2G, KERNEL_SORT = range(6)
 
 # Every exported symbol of include/mpm/mpm.h (checked by the CPU test suite).
@@ -30,6 +32,10 @@ EXPORTS = [
     "mpm_set_timing", "mpm_reset_timers", "mpm_get_kernel_time",
     "mpm_check_finite", "mpm_sort_pairs_host", "mpm_last_error",
     "mpm_batch_create", "mpm_num_scenes", "mpm_scene_range", "mpm_set_graph", "mpm_autotune",
+    "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
+    "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_forward", "mpm_surrogate_gradient",
+    "mpm_surrogate_record", "mpm_surrogate_get_sample", "mpm_surrogate_train", "mpm_surrogate_baseline_loss",
+    "mpm_surrogate_advance",
 ]
 
 
@@ -97,6 +103,19 @@ def load() -> ctypes.CDLL:
         "mpm_scene_range": (i32, [vp, i32, P(i64), P(i64)]),
         "mpm_set_graph": (i32, [vp, i32]),
         "mpm_autotune": (i32, [i32, P(Params), P(CParticles), i32, i32, P(COptions), P(ctypes.c_double)]),
+        "mpm_surrogate_create": (i32, [i32, i32, i32, i32, ctypes.c_uint64, P(vp)]),
+        "mpm_surrogate_destroy": (i32, [vp]),
+        "mpm_surrogate_num_params": (i32, [vp, P(i64)]),
+        "mpm_surrogate_get_params": (i32, [vp, vp]),
+        "mpm_surrogate_set_params": (i32, [vp, vp]),
+        "mpm_surrogate_set_scale": (i32, [vp, ctypes.c_float]),
+        "mpm_surrogate_forward": (i32, [vp, i32, vp, vp, vp]),
+        "mpm_surrogate_gradient": (i32, [vp, i32, vp, vp, vp, vp, vp]),
+        "mpm_surrogate_record": (i32, [vp, vp, i32, i32, i32, i32]),
+        "mpm_surrogate_get_sample": (i32, [vp, i32, vp, vp]),
+        "mpm_surrogate_train": (i32, [vp, vp, i32, ctypes.c_float, vp]),
+        "mpm_surrogate_baseline_loss": (i32, [vp, vp, i32, vp]),
+        "mpm_surrogate_advance": (i32, [vp, vp, i32, i32, ctypes.c_float, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -15,9 +15,11 @@
 
 #include "../../include/mpm/mpm.h"
 #include "mpm_kernels.hpp"
+#include "mpm_solver_impl.hpp"
 #include "mpm_sort.hpp"
 
 using namespace mpm;
+using namespace mpm::detail;
 
 namespace {
 
@@ -36,106 +38,9 @@ int fail(int code, const std::string& msg)
             return fail(MPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
-enum GridState { GRID_CLEAN = 0, GRID_READY = 1, GRID_DIRTY = 2 };
-
-// One set of particle arrays on the device (the solver keeps two for reordering).
-struct ParticleBuffers {
-    void* base = nullptr;
-    float *x = nullptr, *v = nullptr, *C = nullptr, *F = nullptr, *Jp = nullptr;
-    float *mass = nullptr, *vol = nullptr;
-    uint32_t* pid = nullptr;
-};
-
-struct KernelTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-    double total_ms = 0.0;
-    int64_t launches = 0;
-};
-
 }  // namespace
 
-struct mpm_solver {
-    mpm_params params{};
-    mpm_options opts{};
-    Phys ph{};
-    LaunchCfg cfg{};
-    int dim = 2;
-    int n = 0;
-    hipStream_t stream = nullptr;
-
-    ParticleBuffers pb[2];
-    int cur_pb = 0;
-
-    // grids: a[2], vc[2], flags[2]
-    int nbx = 0, nby = 0, nbz = 0, nblocks = 0;
-    float4* ga[2] = {nullptr, nullptr};
-    float4* gvc[2] = {nullptr, nullptr};
-    float4* gf[2] = {nullptr, nullptr};
-    uint32_t* gflags[2] = {nullptr, nullptr};
-    int cur_grid = 0;
-    int grid_state = GRID_CLEAN;
-
-    // binning
-    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr;
-    uint32_t* radix_scratch = nullptr;
-    int key_bits = 0;
-    int4* items = nullptr;
-    int* n_items = nullptr;
-    int max_items = 0;
-    bool items_valid = false;
-    bool sorted_keys_in_alt = false;
-    int since_sort = 0;
-
-    unsigned long long* d_bad = nullptr;
-
-    // batch of independent scenes (nscenes > 1): scene table, block -> scene,
-    // scene particle offsets; a single scene uses none of them
-    int nscenes = 1;
-    std::vector<SceneDesc> scenes_h;
-    std::vector<int64_t> p_off;
-    SceneDesc* scenes_d = nullptr;
-    int* block_scene_d = nullptr;
-    int* p_off_d = nullptr;
-
-    // hipGraph of one binning period pair (2 * sort_every substeps)
-    bool use_graph = false;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_grid = -1, graph_pb = -1, graph_period = 0;
-
-    bool timing = false;
-    KernelTimer timers[MPM_KERNEL_COUNT];
-    std::vector<hipEvent_t> event_pool;
-
-    DevParticles dev_particles() const
-    {
-        const ParticleBuffers& b = pb[cur_pb];
-        DevParticles p;
-        p.x = b.x; p.v = b.v; p.C = b.C; p.F = b.F; p.Jp = b.Jp;
-        p.mass = b.mass; p.vol = b.vol; p.pid = b.pid; p.n = n;
-        return p;
-    }
-    DevGrid dev_grid(int which) const
-    {
-        DevGrid g;
-        g.a = ga[which];
-        g.vc = gvc[which];
-        g.f = gf[which];
-        g.flags = gflags[which];
-        g.nbx = nbx; g.nby = nby; g.nbz = nbz;
-        return g;
-    }
-    hipEvent_t take_event()
-    {
-        if (!event_pool.empty()) {
-            hipEvent_t e = event_pool.back();
-            event_pool.pop_back();
-            return e;
-        }
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-};
+int mpm::detail::set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 namespace {
 
@@ -559,6 +464,7 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
         if (nb_total * 64 >= ((int64_t)1 << 31)) return cleanup_fail(fail(MPM_E_INVALID, "grids too large for 32-bit node addressing"));
         if (nscenes > 1) block_scene.insert(block_scene.end(), (size_t)nb, sc);
         s->scenes_h.push_back(sd);
+        s->scene_params.push_back(P);
         s->p_off.push_back(s->p_off.back() + init[sc].n);
     }
     s->nbx = s->scenes_h[0].nbx;

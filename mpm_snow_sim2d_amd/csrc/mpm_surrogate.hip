@@ -1,0 +1,605 @@
+// mpm_surrogate.hip — the learned grid surrogate of the 2D solver: grid features
+// captured from a solver's particles, a device-resident training set, training
+// steps of the U-Net (mpm_surrogate_net.hip), and the macro step that advances a
+// scene by k substeps with the predicted grid velocity, with the plain solver as
+// fallback.
+//
+// Image: grid nodes (i, j) with 0 <= i < w, 0 <= j < h of a scene whose grid has
+// w = WIDTH cells (nodes 0..WIDTH); the last node row and column lie inside the
+// wall band and are left to the plain solver.
+//
+// Per-node raw features of a sample (FP32 [7][h*w]): mass m_i = sum_p w_ip m_p,
+// velocity v_i = sum_p w_ip m_p v_p / m_i, mean J_P, mean log singular value of F_E
+// (= log|det F_E| / 2), and the Lagrangian target velocity
+//   t_i = sum_p w_ip(t) m_p v_p(t + k dt) / m_i
+// (weights of the positions at capture time, velocities k substeps later).
+// Network input (BF16, 32 channels, normalised): [m/m_ref, v/v_ref, (J-1)*10,
+// logsig*10, occupancy, 0...]; output: the increment over ballistic motion,
+//   v_pred,i = v_i + g k dt e_y + d_ref * out'_i,
+// where out' is projected so that sum_i m_i out'_i = 0 (the prediction changes the
+// momentum of the scene by exactly the gravity impulse).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm/mpm.h"
+#include "mpm_device.hpp"
+#include "mpm_solver_impl.hpp"
+#include "mpm_surrogate_net.hpp"
+#include "mpm_unet.hpp"
+
+using namespace mpm;
+using unet::bf16;
+
+namespace {
+
+constexpr int NCH = 7;           // raw sample channels
+constexpr float V_REF = 3.0f;    // velocity scale of the input (m/s)
+
+struct SlotInfo {
+    float particle_mass;   // sum_p m_p of the scene
+    float grid_mass;       // sum_i m_i over the image
+    float band_mass;       // particle mass within the wall band
+    float T;               // k * dt
+    float gravity;
+    float m_ref;           // RHO * dx^2
+    float pad[2];
+};
+
+int sfail(int code, const std::string& m) { return mpm::detail::set_error(code, m); }
+
+#define S_TRY(expr)                                                                           \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return sfail(MPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__device__ inline void block_add(float v, float* dst, float* sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(dst, sh[0] + sh[1] + sh[2] + sh[3]);
+}
+
+// Scatter of the features (mode 0) or of the target momentum at the saved
+// positions (mode 1) of the particles in slots [first, first + count).
+__global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int count, Phys ph, int h, int w,
+                                                float band, float* __restrict__ sample, float* __restrict__ xs,
+                                                SlotInfo* __restrict__ info, int mode)
+{
+    __shared__ float sh[4];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int n = P.n;
+    const size_t hw = (size_t)h * w;
+    float pm = 0.f, inband = 0.f;
+    if (k < count) {
+        const int p = first + k;
+        const uint32_t id = P.pid[p];
+        float x0 = P.x[p], x1 = P.x[n + p];
+        if (mode == 0) {
+            xs[2 * (size_t)id] = x0;
+            xs[2 * (size_t)id + 1] = x1;
+        } else {
+            x0 = xs[2 * (size_t)id];
+            x1 = xs[2 * (size_t)id + 1];
+        }
+        const float m = P.mass[p];
+        pm = m;
+        const float xg[2] = {x0 * ph.inv_dx, x1 * ph.inv_dx};
+        if (xg[0] < band || xg[0] > (float)ph.W - band || xg[1] < band || xg[1] > (float)ph.H - band) inband = m;
+        Stencil<2, 2> st;
+        st.init(xg);
+        const float v0 = P.v[p], v1 = P.v[n + p];
+        float jp = 0.f, ls = 0.f;
+        if (mode == 0) {
+            jp = P.Jp[p];
+            const float F[4] = {P.F[p], P.F[n + p], P.F[2 * n + p], P.F[3 * n + p]};
+            ls = 0.5f * logf(fmaxf(fabsf(det<2>(F)), 1e-30f));
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int i = st.base[0] + a, j = st.base[1] + b;
+                if (i < 0 || i >= w || j < 0 || j >= h) continue;
+                const float wm = st.w[0][a] * st.w[1][b] * m;
+                const size_t q = (size_t)j * w + i;
+                if (mode == 0) {
+                    atomicAdd(sample + q, wm);
+                    atomicAdd(sample + hw + q, wm * v0);
+                    atomicAdd(sample + 2 * hw + q, wm * v1);
+                    atomicAdd(sample + 3 * hw + q, wm * jp);
+                    atomicAdd(sample + 4 * hw + q, wm * ls);
+                } else {
+                    atomicAdd(sample + 5 * hw + q, wm * v0);
+                    atomicAdd(sample + 6 * hw + q, wm * v1);
+                }
+            }
+    }
+    if (mode == 0) {
+        block_add(pm, &info->particle_mass, sh);
+        block_add(inband, &info->band_mass, sh);
+    }
+}
+
+// Momentum sums to per-node means (channels 1-4 in mode 0, 5-6 in mode 1), and the
+// image's grid mass.
+__global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ sample, int hw, SlotInfo* info,
+                                                       int mode)
+{
+    __shared__ float sh[4];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    float m = 0.f;
+    if (q < hw) {
+        m = sample[q];
+        const float inv = m > 0.f ? 1.f / m : 0.f;
+        if (mode == 0) {
+            for (int c = 1; c <= 4; ++c) sample[(size_t)c * hw + q] *= inv;
+        } else {
+            for (int c = 5; c <= 6; ++c) sample[(size_t)c * hw + q] *= inv;
+        }
+    }
+    if (mode == 0) block_add(m, &info->grid_mass, sh);
+}
+
+// Batch of network inputs, projection weights and targets from dataset slots.
+__global__ void __launch_bounds__(256) k_pack(const float* __restrict__ data, const SlotInfo* __restrict__ info,
+                                             const int* __restrict__ slots, int nb, int hw, float d_ref,
+                                             bf16* __restrict__ in, float* __restrict__ wm,
+                                             float* __restrict__ target)
+{
+    const int64_t n = (int64_t)nb * hw;
+    for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const int b = (int)(t / hw), q = (int)(t - (int64_t)b * hw);
+        const int slot = slots[b];
+        const float* s = data + (size_t)slot * NCH * hw;
+        const SlotInfo I = info[slot];
+        const float m = s[q];
+        const bool occ = m > 0.f;
+        float f[8] = {m / I.m_ref, s[hw + q] / V_REF, s[2 * hw + q] / V_REF,
+                      occ ? (s[3 * (size_t)hw + q] - 1.f) * 10.f : 0.f, s[4 * (size_t)hw + q] * 10.f,
+                      occ ? 1.f : 0.f, 0.f, 0.f};
+        typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+        bf16x8 v0, z;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v0[j] = (bf16)f[j];
+            z[j] = (bf16)0.f;
+        }
+        bf16x8* dst = reinterpret_cast<bf16x8*>(in + (size_t)t * 32);
+        dst[0] = v0;
+        dst[1] = z;
+        dst[2] = z;
+        dst[3] = z;
+        wm[t] = I.grid_mass > 0.f ? m / I.grid_mass : 0.f;
+        if (target) {
+            target[2 * t] = occ ? (s[5 * (size_t)hw + q] - s[hw + q]) / d_ref : 0.f;
+            target[2 * t + 1] = occ ? (s[6 * (size_t)hw + q] - s[2 * (size_t)hw + q] - I.gravity * I.T) / d_ref : 0.f;
+        }
+    }
+}
+
+// Macro step: particles of [first, first+count) advanced by T = k dt with the
+// predicted grid velocity v_i + g T e_y + d_ref * out_i (out == NULL: ballistic).
+__global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, int count, Phys ph, int h, int w,
+                                                  const float* __restrict__ sample, const float* __restrict__ out,
+                                                  float d_ref, float T)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const int p = first + k, n = P.n;
+    const size_t hw = (size_t)h * w;
+    const float xp[2] = {P.x[p], P.x[n + p]};
+    const float xg[2] = {xp[0] * ph.inv_dx, xp[1] * ph.inv_dx};
+    Stencil<2, 2> st;
+    st.init(xg);
+    float nv[2] = {0.f, 0.f}, nB[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int i = st.base[0] + a, j = st.base[1] + b;
+            const float wgt = st.w[0][a] * st.w[1][b];
+            float vi[2] = {0.f, 0.f};
+            if (i >= 0 && i < w && j >= 0 && j < h) {
+                const size_t q = (size_t)j * w + i;
+                vi[0] = sample[hw + q];
+                vi[1] = sample[2 * hw + q];
+                if (out) {
+                    vi[0] += d_ref * out[2 * q];
+                    vi[1] += d_ref * out[2 * q + 1];
+                }
+            } else {
+                vi[0] = P.v[p];
+                vi[1] = P.v[n + p];
+            }
+            vi[1] += ph.gravity * T;
+            const float dp[2] = {st.d[0][a] * ph.dx, st.d[1][b] * ph.dx};
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                nv[r] += wgt * vi[r];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) nB[r * 2 + c] += wgt * (vi[r] * (-dp[c]));
+            }
+        }
+    float Cn[4], M[4], Fo[4], Fh[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        Cn[e] = ph.Dinv * nB[e];
+        M[e] = ((e % 3) == 0 ? 1.f : 0.f) + T * Cn[e];
+        Fo[e] = P.F[e * n + p];
+    }
+    matmul<2>(M, Fo, Fh);
+    float U[4], sig[2], V[4], sc[2];
+    svd<2>(Fh, U, sig, V);
+    float ratio = 1.f;
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        float s = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
+        s = fmaxf(s, 1e-6f);
+        sc[d] = s;
+        ratio *= sig[d] / s;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            P.F[(r * 2 + c) * n + p] = (U[r * 2] * sc[0]) * V[c * 2] + (U[r * 2 + 1] * sc[1]) * V[c * 2 + 1];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        P.x[d * n + p] = xp[d] + 0.5f * T * (P.v[d * n + p] + nv[d]);
+        P.v[d * n + p] = nv[d];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) P.C[e * n + p] = Cn[e];
+    P.Jp[p] = P.Jp[p] * ratio;
+}
+
+__global__ void k_zero_info(SlotInfo* info)
+{
+    info->particle_mass = 0.f;
+    info->grid_mass = 0.f;
+    info->band_mass = 0.f;
+}
+
+__global__ void __launch_bounds__(256) k_zero(float* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.f;
+}
+
+__global__ void k_set_info(SlotInfo* info, float T, float g, float m_ref)
+{
+    info->T = T;
+    info->gravity = g;
+    info->m_ref = m_ref;
+}
+
+__global__ void k_copy_info(SlotInfo* dst, const SlotInfo* src) { *dst = *src; }
+
+}  // namespace
+
+struct mpm_surrogate {
+    hipStream_t st = nullptr;
+    UNet net;
+    int h = 0, w = 0, max_batch = 0, capacity = 0;
+    float d_ref = 0.1f;
+    float* data = nullptr;         // (capacity + 1) slots x NCH x h*w; the last is the live slot
+    SlotInfo* info = nullptr;      // capacity + 1
+    float* xs = nullptr;           // saved positions by original particle index
+    int64_t xs_cap = 0;
+    bf16* in = nullptr;
+    float* wm = nullptr;
+    float* target = nullptr;
+    int* slots_d = nullptr;
+    float* loss_d = nullptr;
+    int live() const { return capacity; }
+};
+
+namespace {
+
+int check_scene(mpm_surrogate* g, mpm_solver* s, int scene)
+{
+    if (!g || !s) return sfail(MPM_E_INVALID, "NULL argument");
+    if (s->dim != 2) return sfail(MPM_E_INVALID, "the surrogate handles 2D scenes");
+    if (scene < 0 || scene >= s->nscenes) return sfail(MPM_E_INVALID, "scene out of range");
+    const mpm_params& P = s->scene_params[scene];
+    if (P.spline_order != 2) return sfail(MPM_E_INVALID, "the surrogate uses the quadratic B-spline");
+    if (P.WIDTH != g->w || P.HEIGHT != g->h)
+        return sfail(MPM_E_INVALID, "scene grid does not match the surrogate image (WIDTH x HEIGHT cells)");
+    return MPM_OK;
+}
+
+int ensure_xs(mpm_surrogate* g, mpm_solver* s)
+{
+    if (g->xs_cap >= s->n) return MPM_OK;
+    if (g->xs) S_TRY(hipFree(g->xs));
+    g->xs = nullptr;
+    g->xs_cap = 0;
+    S_TRY(hipMalloc(&g->xs, (size_t)std::max(s->n, 1) * 2 * sizeof(float)));
+    g->xs_cap = s->n;
+    return MPM_OK;
+}
+
+// Capture into `slot`: mode 0 the features (and saved positions), mode 1 the target.
+int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int k_substeps)
+{
+    int r = check_scene(g, s, scene);
+    if (r) return r;
+    if ((r = ensure_xs(g, s))) return r;
+    // the solver's work is on its own stream: order the capture after it
+    S_TRY(hipStreamSynchronize(s->stream));
+    const mpm_params& P = s->scene_params[scene];
+    const Phys ph = s->scenes_h[scene].ph;
+    const int hw = g->h * g->w;
+    float* sample = g->data + (size_t)slot * NCH * hw;
+    SlotInfo* info = g->info + slot;
+    const int first = (int)s->p_off[scene], count = (int)(s->p_off[scene + 1] - s->p_off[scene]);
+    if (mode == 0) {
+        k_zero<<<std::min((NCH * hw + 255) / 256, 4096), 256, 0, g->st>>>(sample, (int64_t)NCH * hw);
+        k_zero_info<<<1, 1, 0, g->st>>>(info);
+        k_set_info<<<1, 1, 0, g->st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
+    }
+    const float band = P.BORDER_MARGIN + 2.0f;
+    if (count > 0)
+        k_capture<<<(count + 255) / 256, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band,
+                                                         sample, g->xs, info, mode);
+    k_capture_finish<<<(hw + 255) / 256, 256, 0, g->st>>>(sample, hw, info, mode);
+    S_TRY(hipGetLastError());
+    return MPM_OK;
+}
+
+int upload_slots(mpm_surrogate* g, const int* slots, int nb)
+{
+    if (nb < 1 || nb > g->max_batch) return sfail(MPM_E_INVALID, "batch size out of range");
+    for (int b = 0; b < nb; ++b)
+        if (slots[b] < 0 || slots[b] >= g->capacity) return sfail(MPM_E_INVALID, "slot out of range");
+    S_TRY(hipMemcpyAsync(g->slots_d, slots, sizeof(int) * nb, hipMemcpyHostToDevice, g->st));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uint64_t seed, mpm_surrogate** out)
+{
+    if (!out) return sfail(MPM_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (capacity < 0 || max_batch < 1) return sfail(MPM_E_INVALID, "capacity/max_batch out of range");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) return sfail(MPM_E_HIP, "no HIP device");
+    mpm_surrogate* g = new mpm_surrogate();
+    g->h = height;
+    g->w = width;
+    g->max_batch = max_batch;
+    g->capacity = capacity;
+    auto bail = [&](int code) {
+        mpm_surrogate_destroy(g);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(sfail(MPM_E_HIP, "hipStreamCreate failed"));
+    std::string err;
+    int r = g->net.init(height, width, max_batch, seed, g->st, &err);
+    if (r) return bail(sfail(r == -1 ? MPM_E_INVALID : MPM_E_HIP, err));
+    const size_t hw = (size_t)height * width;
+    if (hipMalloc(&g->data, (size_t)(capacity + 1) * NCH * hw * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->info, (size_t)(capacity + 1) * sizeof(SlotInfo)) != hipSuccess ||
+        hipMalloc(&g->in, (size_t)max_batch * hw * 32 * sizeof(bf16)) != hipSuccess ||
+        hipMalloc(&g->wm, (size_t)max_batch * hw * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->target, (size_t)max_batch * hw * 2 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->slots_d, (size_t)max_batch * sizeof(int)) != hipSuccess ||
+        hipMalloc(&g->loss_d, 256) != hipSuccess)
+        return bail(sfail(MPM_E_HIP, "surrogate allocation failed"));
+    if (hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
+        hipStreamSynchronize(g->st) != hipSuccess)
+        return bail(sfail(MPM_E_HIP, "surrogate initialisation failed"));
+    *out = g;
+    return MPM_OK;
+}
+
+int mpm_surrogate_destroy(mpm_surrogate* g)
+{
+    if (!g) return MPM_OK;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    g->net.release();
+    (void)hipFree(g->data);
+    (void)hipFree(g->info);
+    (void)hipFree(g->xs);
+    (void)hipFree(g->in);
+    (void)hipFree(g->wm);
+    (void)hipFree(g->target);
+    (void)hipFree(g->slots_d);
+    (void)hipFree(g->loss_d);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+    return MPM_OK;
+}
+
+int mpm_surrogate_num_params(const mpm_surrogate* g, int64_t* n)
+{
+    if (!g || !n) return sfail(MPM_E_INVALID, "NULL argument");
+    *n = const_cast<mpm_surrogate*>(g)->net.num_params();
+    return MPM_OK;
+}
+
+int mpm_surrogate_get_params(mpm_surrogate* g, float* host)
+{
+    if (!g || !host) return sfail(MPM_E_INVALID, "NULL argument");
+    S_TRY(hipMemcpyAsync(host, g->net.params(), g->net.num_params() * 4, hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    return MPM_OK;
+}
+
+int mpm_surrogate_set_params(mpm_surrogate* g, const float* host)
+{
+    if (!g || !host) return sfail(MPM_E_INVALID, "NULL argument");
+    S_TRY(hipMemcpyAsync(g->net.params(), host, g->net.num_params() * 4, hipMemcpyHostToDevice, g->st));
+    S_TRY(g->net.sync_weights());
+    S_TRY(hipStreamSynchronize(g->st));
+    return MPM_OK;
+}
+
+int mpm_surrogate_set_scale(mpm_surrogate* g, float d_ref)
+{
+    if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    if (!(d_ref > 0.f)) return sfail(MPM_E_INVALID, "d_ref must be > 0");
+    g->d_ref = d_ref;
+    return MPM_OK;
+}
+
+int mpm_surrogate_forward(mpm_surrogate* g, int nb, const uint16_t* input, const float* wm, float* out)
+{
+    if (!g || !input || !wm || !out) return sfail(MPM_E_INVALID, "NULL argument");
+    if (nb < 1 || nb > g->max_batch) return sfail(MPM_E_INVALID, "batch size out of range");
+    const size_t hw = (size_t)g->h * g->w;
+    S_TRY(hipMemcpyAsync(g->in, input, nb * hw * 32 * 2, hipMemcpyHostToDevice, g->st));
+    S_TRY(hipMemcpyAsync(g->wm, wm, nb * hw * 4, hipMemcpyHostToDevice, g->st));
+    S_TRY(g->net.forward(nb, g->in, g->wm));
+    S_TRY(hipMemcpyAsync(out, g->net.out(), nb * hw * 2 * 4, hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    return MPM_OK;
+}
+
+int mpm_surrogate_gradient(mpm_surrogate* g, int nb, const uint16_t* input, const float* wm, const float* target,
+                           float* loss, float* grad)
+{
+    if (!g || !input || !wm || !target) return sfail(MPM_E_INVALID, "NULL argument");
+    if (nb < 1 || nb > g->max_batch) return sfail(MPM_E_INVALID, "batch size out of range");
+    const size_t hw = (size_t)g->h * g->w;
+    S_TRY(hipMemcpyAsync(g->in, input, nb * hw * 32 * 2, hipMemcpyHostToDevice, g->st));
+    S_TRY(hipMemcpyAsync(g->wm, wm, nb * hw * 4, hipMemcpyHostToDevice, g->st));
+    S_TRY(hipMemcpyAsync(g->target, target, nb * hw * 2 * 4, hipMemcpyHostToDevice, g->st));
+    S_TRY(unet::fill_f32(g->loss_d, 1, 0.f, g->st));
+    S_TRY(g->net.loss_and_backward(nb, g->in, g->wm, g->target, g->loss_d, grad != nullptr));
+    if (loss) S_TRY(hipMemcpyAsync(loss, g->loss_d, 4, hipMemcpyDeviceToHost, g->st));
+    if (grad) S_TRY(hipMemcpyAsync(grad, g->net.grads(), g->net.num_params() * 4, hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    return MPM_OK;
+}
+
+int mpm_surrogate_record(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int what, int k_substeps)
+{
+    if (!g) return sfail(MPM_E_INVALID, "surrogate is NULL");
+    if (slot < 0 || slot >= g->capacity) return sfail(MPM_E_INVALID, "slot out of range");
+    if (what != 0 && what != 1) return sfail(MPM_E_INVALID, "what must be 0 (input) or 1 (target)");
+    if (k_substeps < 1) return sfail(MPM_E_INVALID, "k_substeps must be >= 1");
+    return capture(g, s, scene, slot, what, k_substeps);
+}
+
+int mpm_surrogate_get_sample(mpm_surrogate* g, int slot, float* host, float* scalars)
+{
+    if (!g) return sfail(MPM_E_INVALID, "surrogate is NULL");
+    if (slot < 0 || slot > g->capacity) return sfail(MPM_E_INVALID, "slot out of range");
+    const size_t hw = (size_t)g->h * g->w;
+    if (host) S_TRY(hipMemcpyAsync(host, g->data + (size_t)slot * NCH * hw, NCH * hw * 4, hipMemcpyDeviceToHost, g->st));
+    SlotInfo I{};
+    S_TRY(hipMemcpyAsync(&I, g->info + slot, sizeof(I), hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    if (scalars) {
+        scalars[0] = I.particle_mass;
+        scalars[1] = I.grid_mass;
+        scalars[2] = I.band_mass;
+        scalars[3] = I.T;
+        scalars[4] = I.gravity;
+        scalars[5] = I.m_ref;
+    }
+    return MPM_OK;
+}
+
+int mpm_surrogate_train(mpm_surrogate* g, const int* slots, int nb, float lr, float* loss)
+{
+    if (!g || !slots) return sfail(MPM_E_INVALID, "NULL argument");
+    int r = upload_slots(g, slots, nb);
+    if (r) return r;
+    const int hw = g->h * g->w;
+    k_pack<<<std::min((nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, hw, g->d_ref,
+                                                                     g->in, g->wm, g->target);
+    S_TRY(hipGetLastError());
+    S_TRY(unet::fill_f32(g->loss_d, 1, 0.f, g->st));
+    S_TRY(g->net.loss_and_backward(nb, g->in, g->wm, g->target, g->loss_d, lr > 0.f));
+    if (lr > 0.f) S_TRY(g->net.adam(lr));
+    if (loss) {
+        S_TRY(hipMemcpyAsync(loss, g->loss_d, 4, hipMemcpyDeviceToHost, g->st));
+        S_TRY(hipStreamSynchronize(g->st));
+    }
+    return MPM_OK;
+}
+
+int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, float* loss)
+{
+    // the loss of the zero increment (ballistic grid velocity), on the same samples
+    if (!g || !slots || !loss) return sfail(MPM_E_INVALID, "NULL argument");
+    int r = upload_slots(g, slots, nb);
+    if (r) return r;
+    const int hw = g->h * g->w;
+    k_pack<<<std::min((nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, hw, g->d_ref,
+                                                                     g->in, g->wm, g->target);
+    S_TRY(hipGetLastError());
+    std::vector<float> t((size_t)nb * hw * 2), wm((size_t)nb * hw);
+    S_TRY(hipMemcpyAsync(t.data(), g->target, t.size() * 4, hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipMemcpyAsync(wm.data(), g->wm, wm.size() * 4, hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    double l = 0.0;
+    for (size_t i = 0; i < wm.size(); ++i) l += (double)wm[i] * ((double)t[2 * i] * t[2 * i] + (double)t[2 * i + 1] * t[2 * i + 1]);
+    *loss = (float)(l / nb);
+    return MPM_OK;
+}
+
+int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,
+                          float* residual)
+{
+    if (!g || !s) return sfail(MPM_E_INVALID, "NULL argument");
+    if (s->nscenes != 1) return sfail(MPM_E_INVALID, "advance runs on a single-scene solver");
+    if (k_substeps < 1) return sfail(MPM_E_INVALID, "k_substeps must be >= 1");
+    if (mode < MPM_ADVANCE_PLAIN || mode > MPM_ADVANCE_BALLISTIC) return sfail(MPM_E_INVALID, "unknown mode");
+    if (used) *used = MPM_ADVANCE_PLAIN;
+    if (residual) *residual = 0.f;
+    if (mode == MPM_ADVANCE_PLAIN || s->n == 0) return mpm_step(s, k_substeps);
+    int r = capture(g, s, 0, g->live(), 0, k_substeps);
+    if (r) return r;
+    SlotInfo I{};
+    S_TRY(hipMemcpyAsync(&I, g->info + g->live(), sizeof(I), hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipStreamSynchronize(g->st));
+    // share of mass that can exchange momentum with the walls in the next k
+    // substeps, and mass outside the image: the prediction conserves momentum up to
+    // the gravity impulse, so these bound its momentum and mass residuals
+    const float M = I.particle_mass > 0.f ? I.particle_mass : 1.f;
+    const float res = std::max(I.band_mass / M, std::fabs(M - I.grid_mass) / M);
+    if (residual) *residual = res;
+    if (!(res <= tol)) return mpm_step(s, k_substeps);
+
+    const int hw = g->h * g->w;
+    const float* out = nullptr;
+    if (mode == MPM_ADVANCE_NETWORK) {
+        const int live = g->live();
+        S_TRY(hipMemcpyAsync(g->slots_d, &live, sizeof(int), hipMemcpyHostToDevice, g->st));
+        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, 1, hw, g->d_ref,
+                                                                    g->in, g->wm, nullptr);
+        S_TRY(hipGetLastError());
+        S_TRY(g->net.forward(1, g->in, g->wm));
+        out = g->net.out();
+    }
+    const Phys ph = s->scenes_h[0].ph;
+    const float T = (float)k_substeps * ph.dt;
+    const float* sample = g->data + (size_t)g->live() * NCH * hw;
+    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, g->st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
+                                                       g->d_ref, T);
+    S_TRY(hipGetLastError());
+    S_TRY(hipStreamSynchronize(g->st));
+    // the solver's grid and binning no longer match its particles
+    s->grid_state = mpm::detail::GRID_DIRTY;
+    s->items_valid = false;
+    if (used) *used = mode;
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,670 @@
+// mpm_unet.hip — kernels of the grid U-Net surrogate: BF16 matrix-core 3x3
+// convolutions (forward, backward data, backward weights), pooling, upsampling,
+// GELU, the projected velocity head with its loss, and Adam.
+//
+// Matrix-core fragment layout (v_mfma_f32_16x16x32_bf16, wave64), with r = lane % 16
+// and g = lane / 16:
+//   A (16 x 32, rows i, reduction k):  lane holds A[r][8g .. 8g+7]
+//   B (32 x 16, reduction k, cols j):  lane holds B[8g .. 8g+7][r]
+//   D (16 x 16, FP32):                 lane holds D[4g .. 4g+3][r]
+// A and B place the same reduction index in the same (lane group, element) slot,
+// so the product does not depend on how the hardware orders the reduction.
+#include <hip/hip_runtime.h>
+
+#include "mpm_unet.hpp"
+
+namespace mpm {
+namespace unet {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ float gelu(float z)
+{
+    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
+__device__ __forceinline__ bf16x8 zero8()
+{
+    bf16x8 v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (bf16)0.0f;
+    return v;
+}
+
+constexpr int CONV_MT = 2;                 // 16-pixel row tiles per wave
+constexpr int CONV_PIX = 4 * 16 * CONV_MT; // pixels per workgroup
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution as implicit GEMM: rows = pixels, columns = output channels,
+// reduction = (tap, input channel).  Workgroup: 4 waves x 32 pixels, 16*NT output
+// channels (blockIdx.y selects the channel tile).
+template <int NT, bool SPLIT>
+__global__ void __launch_bounds__(256) k_conv3x3(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int cin = a.c1 + a.c2;
+    const int hw = a.h * a.w;
+    const int npix = a.nb * hw;
+    const int co0 = blockIdx.y * 16 * NT;
+    const int pbase = blockIdx.x * CONV_PIX + wave * 16 * CONV_MT;
+
+    int pb[CONV_MT], py[CONV_MT], px[CONV_MT];
+    bool pv[CONV_MT];
+#pragma unroll
+    for (int mt = 0; mt < CONV_MT; ++mt) {
+        const int p = pbase + mt * 16 + r;
+        pv[mt] = p < npix;
+        const int pp = pv[mt] ? p : 0;
+        pb[mt] = pp / hw;
+        const int rem = pp - pb[mt] * hw;
+        py[mt] = rem / a.w;
+        px[mt] = rem - py[mt] * a.w;
+    }
+
+    f32x4 acc[CONV_MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < CONV_MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        int q[CONV_MT];
+        bool ok[CONV_MT];
+#pragma unroll
+        for (int mt = 0; mt < CONV_MT; ++mt) {
+            const int yy = py[mt] + dy, xx = px[mt] + dx;
+            ok[mt] = pv[mt] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+            q[mt] = (pb[mt] * a.h + yy) * a.w + xx;
+        }
+        const bf16* wrow = a.wt + ((size_t)tap * a.cout + co0 + r) * cin + 8 * g;
+        for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+            const bool first = ci0 < a.c1;
+            const bf16* xs = first ? a.x1 : a.x2;
+            const int cs = first ? a.c1 : a.c2;
+            const int cc = (first ? ci0 : ci0 - a.c1) + 8 * g;
+            bf16x8 af[CONV_MT];
+#pragma unroll
+            for (int mt = 0; mt < CONV_MT; ++mt)
+                af[mt] = ok[mt] ? *reinterpret_cast<const bf16x8*>(xs + (size_t)q[mt] * cs + cc) : zero8();
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * t * cin + ci0);
+#pragma unroll
+                for (int mt = 0; mt < CONV_MT; ++mt)
+                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mt], bfr, acc[mt][t], 0, 0, 0);
+            }
+        }
+    }
+
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = co0 + 16 * t + r;
+        const float bias = (!SPLIT && a.bias) ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int mt = 0; mt < CONV_MT; ++mt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int p = pbase + mt * 16 + 4 * g + e;
+                if (p >= npix) continue;
+                const float v = acc[mt][t][e] + bias;
+                if constexpr (!SPLIT) {
+                    if (a.z) a.z[(size_t)p * a.cout + co] = (bf16)v;
+                    if (a.a) a.a[(size_t)p * a.cout + co] = (bf16)gelu(v);
+                } else {
+                    if (co < a.split) {
+                        bf16* d = a.o1 + (size_t)p * a.split + co;
+                        *d = (bf16)(a.add1 ? (float)*d + v : v);
+                    } else {
+                        const int c2n = a.cout - a.split;
+                        bf16* d = a.o2 + (size_t)p * c2n + (co - a.split);
+                        *d = (bf16)(a.add2 ? (float)*d + v : v);
+                    }
+                }
+            }
+    }
+}
+
+template <bool SPLIT>
+hipError_t launch_conv(const ConvArgs& a, hipStream_t st)
+{
+    const int npix = a.nb * a.h * a.w;
+    if (npix == 0) return hipSuccess;
+    const int tiles = a.cout / 16;
+    const int nt = tiles % 8 == 0 ? 8 : tiles % 4 == 0 ? 4 : 2;
+    dim3 grid((npix + CONV_PIX - 1) / CONV_PIX, tiles / nt), blk(256);
+    if (nt == 8) k_conv3x3<8, SPLIT><<<grid, blk, 0, st>>>(a);
+    else if (nt == 4) k_conv3x3<4, SPLIT><<<grid, blk, 0, st>>>(a);
+    else k_conv3x3<2, SPLIT><<<grid, blk, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: per tap a GEMM dW[co][ci] = sum_p dz[p][co] x[p+tap][ci].
+// Workgroup: one tap, a 32x32 (co, ci) tile, a range of pixels; each wave takes
+// 32-pixel slabs, transposes them through its own LDS region ([channel][pixel])
+// and accumulates the 2x2 16x16 output tiles; the four waves reduce through LDS
+// and the workgroup adds its tile into dW with FP32 atomics.
+constexpr int WG_PIX = 2048;   // pixels per workgroup (split of the reduction)
+constexpr int WG_LDS = 40;     // LDS row stride (bf16) of a transposed slab
+
+__global__ void __launch_bounds__(256) k_conv3x3_wgrad(const bf16* __restrict__ x1, const bf16* __restrict__ x2,
+                                                      int c1, int c2, const bf16* __restrict__ dz, int cout,
+                                                      int nb, int h, int w, float* __restrict__ dw)
+{
+    __shared__ __attribute__((aligned(16))) bf16 tz[4][32 * WG_LDS];
+    __shared__ __attribute__((aligned(16))) bf16 tx[4][32 * WG_LDS];
+    __shared__ float red[4][32 * 33];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int cin = c1 + c2;
+    const int ntile_co = cout / 32;
+    const int co_t = blockIdx.x % ntile_co, ci_t = blockIdx.x / ntile_co;
+    const int tap = blockIdx.y;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int hw = h * w, npix = nb * hw;
+    const int p_begin = blockIdx.z * WG_PIX;
+
+    const int ci_abs = ci_t * 32;
+    const bool first = ci_abs < c1;
+    const bf16* xs = first ? x1 : x2;
+    const int cs = first ? c1 : c2;
+    const int cc = first ? ci_abs : ci_abs - c1;
+
+    // staging: lane loads pixel (lane >> 1), channels 16 * (lane & 1) .. +15
+    const int sp = lane >> 1, sc = (lane & 1) * 16;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int s = p_begin + wave * 32; s < p_begin + WG_PIX; s += 128) {
+        const int p = s + sp;
+        bf16x8 z0 = zero8(), z1 = zero8(), v0 = zero8(), v1 = zero8();
+        if (p < npix) {
+            const bf16* zr = dz + (size_t)p * cout + co_t * 32 + sc;
+            z0 = *reinterpret_cast<const bf16x8*>(zr);
+            z1 = *reinterpret_cast<const bf16x8*>(zr + 8);
+            const int b = p / hw, rem = p - b * hw;
+            const int yy = rem / w + dy, xx = rem % w + dx;
+            if (yy >= 0 && yy < h && xx >= 0 && xx < w) {
+                const bf16* xr = xs + (size_t)((b * h + yy) * w + xx) * cs + cc + sc;
+                v0 = *reinterpret_cast<const bf16x8*>(xr);
+                v1 = *reinterpret_cast<const bf16x8*>(xr + 8);
+            }
+        }
+        __syncthreads();
+        bf16* mz = tz[wave];
+        bf16* mx = tx[wave];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            mz[(sc + j) * WG_LDS + sp] = z0[j];
+            mz[(sc + 8 + j) * WG_LDS + sp] = z1[j];
+            mx[(sc + j) * WG_LDS + sp] = v0[j];
+            mx[(sc + 8 + j) * WG_LDS + sp] = v1[j];
+        }
+        __syncthreads();
+        bf16x8 af[2], bfr[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            af[i] = *reinterpret_cast<const bf16x8*>(mz + (16 * i + r) * WG_LDS + 8 * g);
+            bfr[i] = *reinterpret_cast<const bf16x8*>(mx + (16 * i + r) * WG_LDS + 8 * g);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+
+    // D[4g+e][r] of tile (i, j): co = 16i + 4g + e, ci = 16j + r
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) red[wave][(16 * i + 4 * g + e) * 33 + 16 * j + r] = acc[i][j][e];
+    __syncthreads();
+    for (int o = threadIdx.x; o < 1024; o += 256) {
+        const int co = o >> 5, ci = o & 31;
+        const float v = red[0][co * 33 + ci] + red[1][co * 33 + ci] + red[2][co * 33 + ci] + red[3][co * 33 + ci];
+        atomicAdd(dw + ((size_t)tap * cout + co_t * 32 + co) * cin + ci_abs + ci, v);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bias_grad(const bf16* __restrict__ dz, int npix, int cout,
+                                                  float* __restrict__ db)
+{
+    __shared__ float part[256];
+    const int c = threadIdx.x % cout, lanes = 256 / cout, k = threadIdx.x / cout;
+    const int p0 = blockIdx.x * 1024;
+    float s = 0.f;
+    for (int p = p0 + k; p < min(p0 + 1024, npix); p += lanes) s += (float)dz[(size_t)p * cout + c];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int j = 0; j < lanes; ++j) t += part[j * cout + threadIdx.x];
+        atomicAdd(db + threadIdx.x, t);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cast_weights(const float* __restrict__ w, int cout, int cin,
+                                                     bf16* __restrict__ wf, bf16* __restrict__ wb)
+{
+    const int64_t n = (int64_t)9 * cout * cin;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int tap = (int)(i / ((int64_t)cout * cin));
+        const int rem = (int)(i - (int64_t)tap * cout * cin);
+        const int co = rem / cin, ci = rem - co * cin;
+        const bf16 v = (bf16)w[i];
+        wf[i] = v;
+        wb[((size_t)(8 - tap) * cin + ci) * cout + co] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pooling / upsampling (8 channels per thread; c is a multiple of 8)
+__global__ void __launch_bounds__(256) k_avgpool2(const bf16* __restrict__ x, int nb, int h, int w, int c,
+                                                 bf16* __restrict__ y)
+{
+    const int ho = h / 2, wo = w / 2, c8 = c / 8;
+    const int64_t n = (int64_t)nb * ho * wo * c8;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int k = (int)(i % c8);
+        int64_t t = i / c8;
+        const int xo = (int)(t % wo);
+        t /= wo;
+        const int yo = (int)(t % ho);
+        const int b = (int)(t / ho);
+        float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(
+                    x + ((size_t)(b * h + 2 * yo + a) * w + 2 * xo + bb) * c + 8 * k);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+            }
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.25f * s[j]);
+        *reinterpret_cast<bf16x8*>(y + ((size_t)(b * ho + yo) * wo + xo) * c + 8 * k) = o;
+    }
+}
+
+// dx (h x w) from dy (h/2 x w/2): dx = dy / 4 (stored or added)
+__global__ void __launch_bounds__(256) k_avgpool2_bwd(const bf16* __restrict__ dy, int nb, int h, int w, int c,
+                                                     bf16* __restrict__ dx, int add)
+{
+    const int ho = h / 2, wo = w / 2, c8 = c / 8;
+    const int64_t n = (int64_t)nb * h * w * c8;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int k = (int)(i % c8);
+        int64_t t = i / c8;
+        const int x = (int)(t % w);
+        t /= w;
+        const int y = (int)(t % h);
+        const int b = (int)(t / h);
+        const bf16x8 g = *reinterpret_cast<const bf16x8*>(dy + ((size_t)(b * ho + y / 2) * wo + x / 2) * c + 8 * k);
+        bf16x8* d = reinterpret_cast<bf16x8*>(dx + ((size_t)(b * h + y) * w + x) * c + 8 * k);
+        bf16x8 o;
+        if (add) {
+            const bf16x8 old = *d;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = (bf16)((float)old[j] + 0.25f * (float)g[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.25f * (float)g[j]);
+        }
+        *d = o;
+    }
+}
+
+// y (2h x 2w) = nearest upsample of x (h x w)
+__global__ void __launch_bounds__(256) k_upsample2(const bf16* __restrict__ x, int nb, int h, int w, int c,
+                                                  bf16* __restrict__ y)
+{
+    const int h2 = 2 * h, w2 = 2 * w, c8 = c / 8;
+    const int64_t n = (int64_t)nb * h2 * w2 * c8;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int k = (int)(i % c8);
+        int64_t t = i / c8;
+        const int xx = (int)(t % w2);
+        t /= w2;
+        const int yy = (int)(t % h2);
+        const int b = (int)(t / h2);
+        *reinterpret_cast<bf16x8*>(y + ((size_t)(b * h2 + yy) * w2 + xx) * c + 8 * k) =
+            *reinterpret_cast<const bf16x8*>(x + ((size_t)(b * h + yy / 2) * w + xx / 2) * c + 8 * k);
+    }
+}
+
+// dx (h x w) = sum of the four dy (2h x 2w) it was copied to
+__global__ void __launch_bounds__(256) k_upsample2_bwd(const bf16* __restrict__ dy, int nb, int h, int w, int c,
+                                                      bf16* __restrict__ dx)
+{
+    const int h2 = 2 * h, w2 = 2 * w, c8 = c / 8;
+    const int64_t n = (int64_t)nb * h * w * c8;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int k = (int)(i % c8);
+        int64_t t = i / c8;
+        const int x = (int)(t % w);
+        t /= w;
+        const int y = (int)(t % h);
+        const int b = (int)(t / h);
+        float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(
+                    dy + ((size_t)(b * h2 + 2 * y + a) * w2 + 2 * x + bb) * c + 8 * k);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+            }
+        bf16x8 o;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (bf16)s[j];
+        *reinterpret_cast<bf16x8*>(dx + ((size_t)(b * h + y) * w + x) * c + 8 * k) = o;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gelu_bwd(const bf16* __restrict__ da, const bf16* __restrict__ z,
+                                                 int64_t n, bf16* __restrict__ dz)
+{
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dz[i] = (bf16)((float)da[i] * gelu_grad((float)z[i]));
+}
+
+// ---------------------------------------------------------------------------
+// Head (32 -> 2 channels, FP32) and the per-image momentum projection.
+constexpr int HEAD_C = 32;
+
+__device__ __forceinline__ float block_sum(float v, float* sh)
+{
+    // 256 threads: wave reduction then across the four waves
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void __launch_bounds__(256) k_head_forward(const bf16* __restrict__ a, int nb, int hw,
+                                                     const float* __restrict__ wh, const float* __restrict__ bh,
+                                                     const float* __restrict__ wm, float* __restrict__ out,
+                                                     float* __restrict__ sums)
+{
+    __shared__ float sh[4];
+    // blockIdx.y = image; pixels of the image grid-strided over blockIdx.x
+    const int b = blockIdx.y;
+    float s0 = 0.f, s1 = 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += gridDim.x * 256) {
+        const size_t p = (size_t)b * hw + i;
+        const bf16* ar = a + p * HEAD_C;
+        float o0 = bh[0], o1 = bh[1];
+#pragma unroll
+        for (int k = 0; k < HEAD_C / 8; ++k) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(ar + 8 * k);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                o0 += (float)v[j] * wh[8 * k + j];
+                o1 += (float)v[j] * wh[HEAD_C + 8 * k + j];
+            }
+        }
+        out[2 * p] = o0;
+        out[2 * p + 1] = o1;
+        s0 += wm[p] * o0;
+        s1 += wm[p] * o1;
+    }
+    const float t0 = block_sum(s0, sh);
+    const float t1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(sums + 2 * b, t0);
+        atomicAdd(sums + 2 * b + 1, t1);
+    }
+}
+
+// out' = out - sums_b in place; loss and dL/dout'
+__global__ void __launch_bounds__(256) k_head_loss(float* __restrict__ out, const float* __restrict__ target,
+                                                  const float* __restrict__ wm, int nb, int hw,
+                                                  const float* __restrict__ sums, float* __restrict__ loss,
+                                                  float* __restrict__ dproj, float* __restrict__ sums2)
+{
+    __shared__ float sh[4];
+    const int b = blockIdx.y;
+    const float m0 = sums[2 * b], m1 = sums[2 * b + 1];
+    const float inv_nb = 1.0f / (float)nb;
+    float l = 0.f, g0 = 0.f, g1 = 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += gridDim.x * 256) {
+        const size_t p = (size_t)b * hw + i;
+        const float o0 = out[2 * p] - m0, o1 = out[2 * p + 1] - m1;
+        out[2 * p] = o0;
+        out[2 * p + 1] = o1;
+        if (target) {
+            const float d0 = o0 - target[2 * p], d1 = o1 - target[2 * p + 1];
+            const float wmi = wm[p];
+            l += wmi * (d0 * d0 + d1 * d1);
+            if (dproj) {
+                const float q0 = 2.f * wmi * d0 * inv_nb, q1 = 2.f * wmi * d1 * inv_nb;
+                dproj[2 * p] = q0;
+                dproj[2 * p + 1] = q1;
+                g0 += q0;
+                g1 += q1;
+            }
+        }
+    }
+    if (!target) return;
+    const float tl = block_sum(l, sh);
+    const float t0 = block_sum(g0, sh);
+    const float t1 = block_sum(g1, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(loss, tl * inv_nb);
+        if (dproj) {
+            atomicAdd(sums2 + 2 * b, t0);
+            atomicAdd(sums2 + 2 * b + 1, t1);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_head_backward(const bf16* __restrict__ a, const float* __restrict__ dproj,
+                                                      const float* __restrict__ sums2,
+                                                      const float* __restrict__ wm, int nb, int hw,
+                                                      const float* __restrict__ wh, bf16* __restrict__ da,
+                                                      float* __restrict__ dwh, float* __restrict__ dbh)
+{
+    __shared__ float sh[4];
+    const int b = blockIdx.y;
+    const float t0 = sums2[2 * b], t1 = sums2[2 * b + 1];
+    float gw[2 * HEAD_C];
+#pragma unroll
+    for (int k = 0; k < 2 * HEAD_C; ++k) gw[k] = 0.f;
+    float gb0 = 0.f, gb1 = 0.f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += gridDim.x * 256) {
+        const size_t p = (size_t)b * hw + i;
+        // projection adjoint: dout_j = dproj_j - wm_j * sum_i dproj_i
+        const float d0 = dproj[2 * p] - wm[p] * t0, d1 = dproj[2 * p + 1] - wm[p] * t1;
+        gb0 += d0;
+        gb1 += d1;
+        const bf16* ar = a + p * HEAD_C;
+        bf16* dr = da + p * HEAD_C;
+#pragma unroll
+        for (int k = 0; k < HEAD_C / 8; ++k) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(ar + 8 * k);
+            bf16x8 o;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = 8 * k + j;
+                o[j] = (bf16)(d0 * wh[c] + d1 * wh[HEAD_C + c]);
+                gw[c] += d0 * (float)v[j];
+                gw[HEAD_C + c] += d1 * (float)v[j];
+            }
+            *reinterpret_cast<bf16x8*>(dr + 8 * k) = o;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * HEAD_C; ++k) {
+        const float t = block_sum(gw[k], sh);
+        if (threadIdx.x == 0) atomicAdd(dwh + k, t);
+    }
+    const float s0 = block_sum(gb0, sh);
+    const float s1 = block_sum(gb1, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(dbh, s0);
+        atomicAdd(dbh + 1, s1);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                             float* __restrict__ m, float* __restrict__ v, int64_t n, float lr_t,
+                                             float beta1, float beta2, float eps_t)
+{
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+        const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr_t * mi / (sqrtf(vi) + eps_t);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_f32(float* p, int64_t n, float value)
+{
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = value;
+}
+
+__global__ void __launch_bounds__(256) k_fill_bf16(bf16* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = (bf16)0.0f;
+}
+
+inline unsigned grid_for(int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+inline unsigned head_blocks(int hw)
+{
+    const int b = (hw + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 64 ? 64 : b));
+}
+
+}  // namespace
+
+hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st) { return launch_conv<false>(a, st); }
+hipError_t conv3x3_split(const ConvArgs& a, hipStream_t st) { return launch_conv<true>(a, st); }
+
+hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, const bf16* dz, int cout, int nb,
+                         int h, int w, float* dw, hipStream_t st)
+{
+    const int npix = nb * h * w;
+    if (npix == 0) return hipSuccess;
+    dim3 grid((cout / 32) * ((c1 + c2) / 32), 9, (npix + WG_PIX - 1) / WG_PIX), blk(256);
+    k_conv3x3_wgrad<<<grid, blk, 0, st>>>(x1, x2, c1, c2, dz, cout, nb, h, w, dw);
+    return hipGetLastError();
+}
+
+hipError_t bias_grad(const bf16* dz, int npix, int cout, float* db, hipStream_t st)
+{
+    if (npix == 0) return hipSuccess;
+    k_bias_grad<<<(npix + 1023) / 1024, 256, 0, st>>>(dz, npix, cout, db);
+    return hipGetLastError();
+}
+
+hipError_t cast_weights(const float* w, int cout, int cin, bf16* wf, bf16* wb, hipStream_t st)
+{
+    k_cast_weights<<<grid_for((int64_t)9 * cout * cin), 256, 0, st>>>(w, cout, cin, wf, wb);
+    return hipGetLastError();
+}
+
+hipError_t avgpool2(const bf16* x, int nb, int h, int w, int c, bf16* y, hipStream_t st)
+{
+    k_avgpool2<<<grid_for((int64_t)nb * (h / 2) * (w / 2) * (c / 8)), 256, 0, st>>>(x, nb, h, w, c, y);
+    return hipGetLastError();
+}
+
+hipError_t avgpool2_bwd(const bf16* dy, int nb, int h, int w, int c, bf16* dx, int add, hipStream_t st)
+{
+    k_avgpool2_bwd<<<grid_for((int64_t)nb * h * w * (c / 8)), 256, 0, st>>>(dy, nb, h, w, c, dx, add);
+    return hipGetLastError();
+}
+
+hipError_t upsample2(const bf16* x, int nb, int h, int w, int c, bf16* y, hipStream_t st)
+{
+    k_upsample2<<<grid_for((int64_t)nb * 4 * h * w * (c / 8)), 256, 0, st>>>(x, nb, h, w, c, y);
+    return hipGetLastError();
+}
+
+hipError_t upsample2_bwd(const bf16* dy, int nb, int h, int w, int c, bf16* dx, hipStream_t st)
+{
+    k_upsample2_bwd<<<grid_for((int64_t)nb * h * w * (c / 8)), 256, 0, st>>>(dy, nb, h, w, c, dx);
+    return hipGetLastError();
+}
+
+hipError_t gelu_bwd(const bf16* da, const bf16* z, int64_t n, bf16* dz, hipStream_t st)
+{
+    k_gelu_bwd<<<grid_for(n), 256, 0, st>>>(da, z, n, dz);
+    return hipGetLastError();
+}
+
+hipError_t head_forward(const bf16* a, int nb, int hw, const float* wh, const float* bh, const float* wm,
+                        float* out, float* sums, hipStream_t st)
+{
+    k_fill_f32<<<1, 256, 0, st>>>(sums, 2 * nb, 0.f);
+    dim3 grid(head_blocks(hw), nb);
+    k_head_forward<<<grid, 256, 0, st>>>(a, nb, hw, wh, bh, wm, out, sums);
+    return hipGetLastError();
+}
+
+hipError_t head_project_loss(float* out, const float* sums, const float* target, const float* wm, int nb,
+                             int hw, float* loss, float* dproj, float* sums2, hipStream_t st)
+{
+    if (target && dproj) k_fill_f32<<<1, 256, 0, st>>>(sums2, 2 * nb, 0.f);
+    dim3 grid(head_blocks(hw), nb);
+    k_head_loss<<<grid, 256, 0, st>>>(out, target, wm, nb, hw, sums, loss, dproj, sums2);
+    return hipGetLastError();
+}
+
+hipError_t head_backward(const bf16* a, const float* dproj, const float* sums2, const float* wm, int nb,
+                         int hw, const float* wh, bf16* da, float* dwh, float* dbh, hipStream_t st)
+{
+    dim3 grid(head_blocks(hw), nb);
+    k_head_backward<<<grid, 256, 0, st>>>(a, dproj, sums2, wm, nb, hw, wh, da, dwh, dbh);
+    return hipGetLastError();
+}
+
+hipError_t adam_step(float* w, const float* g, float* m, float* v, int64_t n, float lr_t, float beta1,
+                     float beta2, float eps_t, hipStream_t st)
+{
+    k_adam<<<grid_for(n), 256, 0, st>>>(w, g, m, v, n, lr_t, beta1, beta2, eps_t);
+    return hipGetLastError();
+}
+
+hipError_t fill_f32(float* p, int64_t n, float value, hipStream_t st)
+{
+    k_fill_f32<<<grid_for(n), 256, 0, st>>>(p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t fill_bf16(bf16* p, int64_t n, hipStream_t st)
+{
+    k_fill_bf16<<<grid_for(n), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace unet
+}  // namespace mpm

@@ -274,3 +274,40 @@ def batch_scenes(nscenes: int = 64, seed: int = 0, n_min: int = 5_000, n_max: in
         cells, dt = (128, 1e-4) if n < split else (256, 5e-5)
         out.append((snow_params(2, cells, dt), random_scene(seed * 100_003 + i, int(n))))
     return out
+
+
+def colliding_scene(seed: int, n_target: int, rho: float = 400.0, max_shapes: int = 3,
+                    speed=(1.0, 3.0)) -> ParticleSet:
+    """Random 2D scene of interacting bodies: 2..max_shapes disks (r ~ U(0.05, 0.10))
+    or boxes (half-sides ~ U(0.04, 0.08)) at seeded non-overlapping centres in
+    [0.25, 0.75]^2, each moving towards the middle of the square at U(speed) m/s
+    plus a U(-0.5, 0.5) m/s perturbation per axis; one jittered lattice of about
+    n_target particles in the union."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, max_shapes + 1))
+    shapes = []
+    attempts = 0
+    while len(shapes) < k and attempts < 1000:
+        attempts += 1
+        box = bool(rng.integers(0, 2))
+        c = rng.uniform(0.25, 0.75, 2)
+        half = rng.uniform(0.04, 0.08, 2) if box else np.full(2, rng.uniform(0.05, 0.10))
+        reach = float(np.hypot(*half)) if box else float(half[0])
+        if all(np.linalg.norm(c - c2) > reach + r2 + 0.005 for _, c2, _, r2 in shapes):
+            shapes.append((box, c, half, reach))
+    area = sum(4.0 * h[0] * h[1] if box else math.pi * h[0] ** 2 for box, _, h, _ in shapes)
+    h = math.sqrt(area / n_target)
+    out = ParticleSet.empty(2)
+    mid = np.array([0.5, 0.5])
+    for box, c, half, _ in shapes:
+        if box:
+            inside = lambda p, c=c, half=half: (np.abs(p - c) <= half).all(1)
+        else:
+            inside = lambda p, c=c, r=half[0]: ((p - c) ** 2).sum(1) <= r * r
+        pts = jitter_lattice(c - half, c + half, h, rng, inside)
+        d = mid - c
+        d = d / max(np.linalg.norm(d), 1e-9)
+        vel = d * rng.uniform(*speed) + rng.uniform(-0.5, 0.5, 2)
+        out = out.concat(_set(pts, vel, h * h, rho * h * h))
+    out.meta = {"scene": "colliding_scene", "seed": seed, "h": h, "shapes": len(shapes), "n_target": n_target}
+    return out

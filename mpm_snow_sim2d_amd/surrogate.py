@@ -1,0 +1,214 @@
+"""Learned grid surrogate over the C-ABI (``mpm_surrogate_*``): a BF16 U-Net on the
+matrix cores that predicts the grid velocity k substeps ahead, trained on the
+device from trajectories the solver generates, and a hybrid rollout that uses it
+where its residual test passes and the plain solver elsewhere.
+
+``Surrogate`` wraps one network with its device-resident training set;
+``generate`` fills the set from a batch of scenes; ``fit`` runs Adam steps;
+``rollout`` advances a single-scene solver in macro steps of k substeps.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+ADVANCE_PLAIN = _lib.ADVANCE_PLAIN
+ADVANCE_NETWORK = _lib.ADVANCE_NETWORK
+ADVANCE_BALLISTIC = _lib.ADVANCE_BALLISTIC
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Surrogate:
+    """One U-Net (height x width grid nodes) and `capacity` training slots."""
+
+    def __init__(self, height: int, width: int, max_batch: int = 8, capacity: int = 0, seed: int = 0,
+                 d_ref: float = 0.1):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.mpm_surrogate_create(int(height), int(width), int(max_batch), int(capacity),
+                                             ctypes.c_uint64(seed), ctypes.byref(h)))
+        self._h = h
+        self.h, self.w, self.max_batch, self.capacity = height, width, max_batch, capacity
+        self.set_scale(d_ref)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.mpm_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- parameters
+    @property
+    def num_params(self) -> int:
+        n = ctypes.c_int64(0)
+        check(self._lib.mpm_surrogate_num_params(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.num_params, np.float32)
+        check(self._lib.mpm_surrogate_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise ValueError("parameter vector has the wrong size")
+        check(self._lib.mpm_surrogate_set_params(self._h, _ptr(p)))
+
+    def set_scale(self, d_ref: float) -> None:
+        self.d_ref = float(d_ref)
+        check(self._lib.mpm_surrogate_set_scale(self._h, ctypes.c_float(d_ref)))
+
+    # -- the network on host arrays
+    def forward(self, input_bits: np.ndarray, wm: np.ndarray) -> np.ndarray:
+        nb = input_bits.shape[0]
+        x = np.ascontiguousarray(input_bits, np.uint16)
+        w = np.ascontiguousarray(wm, np.float32)
+        out = np.empty((nb, self.h * self.w, 2), np.float32)
+        check(self._lib.mpm_surrogate_forward(self._h, nb, _ptr(x), _ptr(w), _ptr(out)))
+        return out
+
+    def gradient(self, input_bits: np.ndarray, wm: np.ndarray, target: np.ndarray, grad: bool = True):
+        nb = input_bits.shape[0]
+        x = np.ascontiguousarray(input_bits, np.uint16)
+        w = np.ascontiguousarray(wm, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.empty(self.num_params, np.float32) if grad else None
+        check(self._lib.mpm_surrogate_gradient(self._h, nb, _ptr(x), _ptr(w), _ptr(t), _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    # -- training set
+    def record(self, solver, scene: int, slot: int, what: int, k: int) -> None:
+        check(self._lib.mpm_surrogate_record(self._h, solver._h, int(scene), int(slot), int(what), int(k)))
+
+    def get_sample(self, slot: int):
+        data = np.empty((7, self.h * self.w), np.float32)
+        sc = np.empty(6, np.float32)
+        check(self._lib.mpm_surrogate_get_sample(self._h, int(slot), _ptr(data), _ptr(sc)))
+        keys = ("particle_mass", "grid_mass", "band_mass", "T", "gravity", "m_ref")
+        return data, dict(zip(keys, map(float, sc)))
+
+    def train(self, slots, lr: float) -> float:
+        s = np.ascontiguousarray(slots, np.int32)
+        loss = np.zeros(1, np.float32)
+        check(self._lib.mpm_surrogate_train(self._h, _ptr(s), s.size, ctypes.c_float(lr), _ptr(loss)))
+        return float(loss[0])
+
+    def loss(self, slots) -> float:
+        return self.train(slots, 0.0)
+
+    def baseline_loss(self, slots) -> float:
+        s = np.ascontiguousarray(slots, np.int32)
+        loss = np.zeros(1, np.float32)
+        check(self._lib.mpm_surrogate_baseline_loss(self._h, _ptr(s), s.size, _ptr(loss)))
+        return float(loss[0])
+
+    # -- rollout
+    def advance(self, solver, k: int, mode: int, tol: float = 1e-3) -> tuple[int, float]:
+        used = ctypes.c_int(0)
+        res = ctypes.c_float(0.0)
+        check(self._lib.mpm_surrogate_advance(self._h, solver._h, int(k), int(mode), ctypes.c_float(tol),
+                                              ctypes.byref(used), ctypes.byref(res)))
+        return int(used.value), float(res.value)
+
+
+def generate(sur: Surrogate, scenes, substeps: int, k: int = 8, first_slot: int = 0, options=None,
+             graph: bool = True, wall_tol: float | None = None) -> list[list[int]]:
+    """Run the scenes (list of (Params, ParticleSet), all on the surrogate's grid)
+    together for `substeps` substeps and record one (input, target) sample per scene
+    every k substeps.  Returns the slots of each scene, in time order.  With
+    wall_tol, samples whose wall-band mass share exceeds it (where the rollout falls
+    back to the plain solver, and the target holds wall impulses the projected
+    prediction cannot represent) are left out of the returned lists."""
+    from .params import Options
+    from .solver import BatchSolver
+    nmacro = substeps // k
+    need = len(scenes) * nmacro
+    if first_slot + need > sur.capacity:
+        raise ValueError(f"{need} samples do not fit in the {sur.capacity - first_slot} free slots")
+    slots = [[] for _ in scenes]
+    with BatchSolver(scenes, options or Options(sort_every=k, tile_particles=256), graph=graph) as b:
+        slot = first_slot
+        for _ in range(nmacro):
+            for i in range(len(scenes)):
+                sur.record(b, i, slot + i, 0, k)
+            b.Step(k)
+            for i in range(len(scenes)):
+                sur.record(b, i, slot + i, 1, k)
+                if wall_tol is not None:
+                    _, sc = sur.get_sample(slot + i)
+                    pm = max(sc["particle_mass"], 1e-30)
+                    if sc["band_mass"] / pm > wall_tol or abs(pm - sc["grid_mass"]) / pm > wall_tol:
+                        continue
+                slots[i].append(slot + i)
+            slot += len(scenes)
+    return slots
+
+
+def fit(sur: Surrogate, train_slots, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
+        lr_final: float | None = None, log_every: int = 0, log=print) -> list[float]:
+    """Adam steps on random batches of the training slots (cosine decay of the
+    learning rate to lr_final)."""
+    rng = np.random.default_rng(seed)
+    pool = np.asarray(train_slots, np.int32)
+    lr_final = lr * 0.1 if lr_final is None else lr_final
+    losses = []
+    for it in range(steps):
+        lr_t = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / max(steps - 1, 1)))
+        sel = rng.choice(pool, size=min(batch, pool.size), replace=False)
+        want = log_every and (it % log_every == 0 or it == steps - 1)
+        if want:
+            losses.append(sur.train(sel, lr_t))
+            log(f"[fit] step {it} loss {losses[-1]:.5f} lr {lr_t:.2e}")
+        else:
+            sur.train(sel, lr_t)
+    return losses
+
+
+def rollout(sur: Surrogate | None, params, ps, substeps: int, k: int = 8, mode: int = ADVANCE_NETWORK,
+            tol: float = 1e-3, options=None, graph: bool = True) -> dict:
+    """Advance one scene by `substeps` (a multiple of k) in macro steps; mode
+    ADVANCE_PLAIN runs the plain solver only.  Returns the final positions
+    (n, 2), the wall time, and how many macro steps each mode took."""
+    from .params import Options
+    from .solver import GPUSolver
+    s = GPUSolver(params, ps, options or Options(sort_every=k, tile_particles=256))
+    if graph:
+        s.set_graph(True)
+    counts = {ADVANCE_PLAIN: 0, ADVANCE_NETWORK: 0, ADVANCE_BALLISTIC: 0}
+    s.sync()
+    t0 = time.perf_counter()
+    if mode == ADVANCE_PLAIN or sur is None:
+        s.Step(substeps // k * k)
+        counts[ADVANCE_PLAIN] = substeps // k
+    else:
+        for _ in range(substeps // k):
+            used, _ = sur.advance(s, k, mode, tol)
+            counts[used] += 1
+    s.sync()
+    wall = time.perf_counter() - t0
+    x = s.GetParticlePositions().copy()
+    bad = s.check_finite()
+    s.close()
+    return {"x": x, "wall": wall, "counts": counts, "nonfinite": bad}

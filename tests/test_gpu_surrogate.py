@@ -1,0 +1,262 @@
+"""The learned surrogate on the device, through the C-ABI.
+
+* Network: forward output and parameter gradients of the BF16 matrix-core U-Net
+  against the FP64 oracle (oracle/unet_ref.py) fed the same BF16-rounded weights
+  and inputs.  Bounds, relative L2 over each tensor: output 1e-2, loss 1e-2, every
+  parameter block's gradient 2e-2 with cosine similarity >= 0.9995 (BF16 keeps 8
+  significant bits; activations and activation gradients are rounded to it after
+  every layer); the head bias, on which the projected loss does not depend, has a
+  gradient of zero up to rounding.
+* Capture: node features and Lagrangian targets against a NumPy P2G of the same
+  particles (FP32 atomics against FP64 sums: relative 1e-5).
+* Macro step: ballistic advance of a stress-free body in free fall against its
+  closed form, fallback to plain substeps at the walls.
+* Ablation: a network trained on solver trajectories predicts the velocity of
+  held-out colliding scenes better than the method without it (the ballistic grid
+  velocity, i.e. a zero network output), and its hybrid rollout lands closer to
+  the plain solver.
+"""
+import numpy as np
+import pytest
+
+from mpm_snow_sim2d_amd.params import Options, snow_params
+from mpm_snow_sim2d_amd.scenes import ParticleSet, colliding_scene, jitter_lattice, _set
+from oracle import unet_ref as U
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _net_problem(rng, sur, nb, h, w):
+    params = sur.get_params().astype(np.float64)
+    p = U.unpack(params.copy())
+    for name in U.NAMES:      # the device convolves with BF16 copies of the weights
+        p[name + ".w"] = U.bf16_round(p[name + ".w"])
+    x = rng.normal(size=(nb, h, w, 32)).astype(np.float32)
+    x[..., 8:] = 0.0
+    bits = U.to_bf16_bits(x).reshape(nb, h, w, 32)
+    xr = U.from_bf16_bits(bits).astype(np.float64)
+    m = rng.uniform(0.0, 1.0, size=(nb, h * w)) * (rng.uniform(size=(nb, h * w)) > 0.4)
+    wm = (m / m.sum(1, keepdims=True)).astype(np.float32)
+    target = rng.normal(scale=0.5, size=(nb, h * w, 2)).astype(np.float32)
+    return p, bits, xr, wm, target
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_network_forward_matches_oracle():
+    from mpm_snow_sim2d_amd.surrogate import Surrogate
+    rng = np.random.default_rng(0)
+    nb, h, w = 2, 32, 32
+    _flush("[surrogate] forward 2x32x32")
+    with Surrogate(h, w, max_batch=nb, seed=1) as sur:
+        # a head large enough that the output is not dominated by rounding
+        prm = sur.get_params()
+        o, _ = U.layout()[0]["head.w"]
+        prm[o:o + 64] = rng.normal(scale=0.5, size=64)
+        sur.set_params(prm)
+        p, bits, xr, wm, _ = _net_problem(rng, sur, nb, h, w)
+        out = sur.forward(bits, wm)
+    ref = U.forward(p, xr, wm.astype(np.float64))
+    err = _rel(out, ref)
+    _flush(f"    rel L2 {err:.3e}")
+    assert err < 1e-2
+
+
+def test_network_gradient_matches_oracle():
+    from mpm_snow_sim2d_amd.surrogate import Surrogate
+    rng = np.random.default_rng(1)
+    nb, h, w = 2, 32, 32
+    _flush("[surrogate] gradient 2x32x32")
+    with Surrogate(h, w, max_batch=nb, seed=2) as sur:
+        prm = sur.get_params()
+        o, _ = U.layout()[0]["head.w"]
+        prm[o:o + 64] = rng.normal(scale=0.5, size=64)
+        sur.set_params(prm)
+        p, bits, xr, wm, target = _net_problem(rng, sur, nb, h, w)
+        loss, g = sur.gradient(bits, wm, target)
+    lref, gref = U.loss_and_grad(p, xr, wm.astype(np.float64), target.astype(np.float64))
+    _flush(f"    loss {loss:.6f} oracle {lref:.6f}")
+    assert abs(loss - lref) <= 1e-2 * abs(lref)
+    gd = U.unpack(g.astype(np.float64))
+    # the projection removes any constant from the output, so the loss does not
+    # depend on the head bias: its gradient vanishes identically
+    assert np.abs(gd["head.b"]).max() <= 1e-5 * np.linalg.norm(gd["head.w"])
+    for k in gref:
+        if k == "head.b":
+            continue
+        a, b = gd[k].reshape(-1), gref[k].reshape(-1)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        err = _rel(a, b)
+        _flush(f"    {k:8s} rel {err:.3e} cos {cos:.5f}")
+        assert err < 2e-2 and cos >= 0.9995, k
+
+
+def test_training_steps_reduce_loss():
+    """Adam on the device fits a fixed batch of captured samples."""
+    from mpm_snow_sim2d_amd.surrogate import Surrogate, generate
+    P = snow_params(2, 32, 4e-4)
+    scenes = [(P, colliding_scene(50 + i, 1500)) for i in range(2)]
+    with Surrogate(32, 32, max_batch=4, capacity=24, seed=3) as sur:
+        # the last samples of 96 substeps, after the bodies have met
+        slots = [ss[-2:] for ss in generate(sur, scenes, 96, 8)]
+        slots = [s for ss in slots for s in ss]
+        l0 = sur.loss(slots)
+        assert l0 > 1e-3
+        for _ in range(60):
+            sur.train(slots, 2e-3)
+        l1 = sur.loss(slots)
+    _flush(f"[surrogate] fixed-batch loss {l0:.4f} -> {l1:.4f}")
+    assert np.isfinite(l1) and l1 < 0.5 * l0
+
+
+def _p2g_numpy(P, x, vals, m, h, w):
+    """sum_p w_ip m_p vals_p over nodes [0,w) x [0,h) (quadratic B-spline, FP64)."""
+    inv = 1.0 / P.dx
+    acc = np.zeros((vals.shape[0] + 1, h * w))
+    xg = x.astype(np.float64) * inv
+    base = np.floor(xg - 0.5).astype(np.int64)
+    for a in range(3):
+        for b in range(3):
+            i, j = base[0] + a, base[1] + b
+            di, dj = np.abs(xg[0] - i), np.abs(xg[1] - j)
+            wx = np.where(di < 0.5, 0.75 - di * di, np.where(di < 1.5, 0.5 * (1.5 - di) ** 2, 0.0))
+            wy = np.where(dj < 0.5, 0.75 - dj * dj, np.where(dj < 1.5, 0.5 * (1.5 - dj) ** 2, 0.0))
+            wgt = wx * wy * m
+            ok = (i >= 0) & (i < w) & (j >= 0) & (j < h)
+            q = (j * w + i)[ok]
+            np.add.at(acc[0], q, wgt[ok])
+            for c in range(vals.shape[0]):
+                np.add.at(acc[c + 1], q, (wgt * vals[c])[ok])
+    return acc
+
+
+def test_capture_matches_numpy():
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.surrogate import Surrogate
+    P = snow_params(2, 64, 2e-4)
+    ps = colliding_scene(5, 3000)
+    k = 8
+    _flush(f"[surrogate] capture n={ps.n}")
+    with Surrogate(64, 64, max_batch=1, capacity=2) as sur:
+        s = GPUSolver(P, ps, Options(sort_every=4, tile_particles=256))
+        s.Step(6)
+        st0 = s.get_state()
+        sur.record(s, 0, 1, 0, k)
+        s.Step(k)
+        st1 = s.get_state()
+        sur.record(s, 0, 1, 1, k)
+        data, sc = sur.get_sample(1)
+        s.close()
+    m = ps.mass.astype(np.float64)
+    F = st0["F"].astype(np.float64)
+    ls = 0.5 * np.log(np.abs(F[0] * F[3] - F[1] * F[2]))
+    acc = _p2g_numpy(P, st0["x"], np.stack([st0["v"][0], st0["v"][1], st0["Jp"], ls]), m, 64, 64)
+    tgt = _p2g_numpy(P, st0["x"], st1["v"].astype(np.float64), m, 64, 64)
+    mass = acc[0]
+    occ = mass > 1e-6 * mass.max()
+    assert np.allclose(data[0], mass, rtol=1e-5, atol=1e-6 * mass.max())
+    for c in range(4):
+        ref = acc[c + 1][occ] / mass[occ]
+        # log|det F_E| / 2 of a nearly undeformed body is ~1e-4: FP32 rounding of
+        # det F_E (6e-8) bounds it absolutely
+        tol = max(1e-4 * np.abs(ref).max(), 1e-6)
+        assert np.abs(data[c + 1][occ] - ref).max() <= tol, c
+    for c in range(2):
+        ref = tgt[c + 1][occ] / mass[occ]
+        assert np.abs(data[5 + c][occ] - ref).max() <= 1e-4 * np.abs(ref).max(), c
+    assert abs(sc["particle_mass"] - m.sum()) <= 1e-5 * m.sum()
+    assert abs(sc["T"] - k * np.float32(P.DELTA_TIME)) < 1e-9
+    assert sc["band_mass"] == 0.0
+
+
+def test_ballistic_macro_step_free_fall():
+    """A stress-free body with uniform velocity: v' = v + g T, x' = x + T v + g T^2/2,
+    F and J_P unchanged."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.surrogate import ADVANCE_BALLISTIC, Surrogate
+    P = snow_params(2, 64, 2e-4)
+    rng = np.random.default_rng(4)
+    c = np.array([0.5, 0.55])
+    pts = jitter_lattice(c - 0.08, c + 0.08, 0.004, rng, lambda p: ((p - c) ** 2).sum(1) <= 0.08 ** 2)
+    ps = _set(pts, (0.7, -0.3), 0.004 ** 2, 400 * 0.004 ** 2)
+    k = 8
+    with Surrogate(64, 64, max_batch=1, capacity=0) as sur:
+        s = GPUSolver(P, ps, Options(sort_every=4))
+        st0 = s.get_state()
+        used, res = sur.advance(s, k, ADVANCE_BALLISTIC, 1e-3)
+        st1 = s.get_state()
+        s.close()
+    assert used == ADVANCE_BALLISTIC and res <= 1e-3
+    T = k * np.float32(P.DELTA_TIME)
+    g = np.array([0.0, P.GRAVITY])
+    v0 = st0["v"].astype(np.float64)
+    assert np.abs(st1["v"] - (v0 + g[:, None] * T)).max() < 1e-5
+    xr = st0["x"] + T * v0 + 0.5 * g[:, None] * T * T
+    assert np.abs(st1["x"] - xr).max() < 1e-6
+    assert np.abs(st1["F"] - st0["F"]).max() < 1e-5
+    assert np.abs(st1["Jp"] - st0["Jp"]).max() < 1e-6
+
+
+def test_advance_falls_back_at_walls():
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.surrogate import ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate
+    P = snow_params(2, 64, 2e-4)
+    rng = np.random.default_rng(5)
+    c = np.array([0.5, 0.1])
+    pts = jitter_lattice(c - 0.06, c + 0.06, 0.004, rng, lambda p: ((p - c) ** 2).sum(1) <= 0.06 ** 2)
+    ps = _set(pts, (0.0, -1.0), 0.004 ** 2, 400 * 0.004 ** 2)
+    k = 8
+    with Surrogate(64, 64, max_batch=1, capacity=0) as sur:
+        a = GPUSolver(P, ps, Options(sort_every=4))
+        b = GPUSolver(P, ps, Options(sort_every=4))
+        used, res = sur.advance(a, k, ADVANCE_NETWORK, 1e-3)
+        b.Step(k)
+        sa, sb = a.get_state(), b.get_state()
+        a.close()
+        b.close()
+    assert used == ADVANCE_PLAIN and res > 1e-3
+    assert np.abs(sa["x"] - sb["x"]).max() < 1e-6
+
+
+def test_trained_network_beats_ballistic():
+    """Ablation: train on trajectories of colliding scenes (64^2 grid), then on
+    held-out scenes compare the network with the zero-output (ballistic) method:
+    lower velocity loss, and a hybrid rollout whose positions land closer to the
+    plain solver's."""
+    from mpm_snow_sim2d_amd.surrogate import (ADVANCE_BALLISTIC, ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate, fit,
+                                              generate, rollout)
+    cells, dt, k = 64, 2e-4, 8
+    P = snow_params(2, cells, dt)
+    train_scenes = [(P, colliding_scene(100 + i, 4000)) for i in range(48)]
+    test_scenes = [(P, colliding_scene(900 + i, 4000)) for i in range(6)]
+    nsub = 240
+    cap = (len(train_scenes) + len(test_scenes)) * (nsub // k)
+    with Surrogate(cells, cells, max_batch=16, capacity=cap, seed=7, d_ref=0.1) as sur:
+        tr = generate(sur, train_scenes, nsub, k, wall_tol=1e-3)
+        te = generate(sur, test_scenes, nsub, k, first_slot=len(train_scenes) * (nsub // k), wall_tol=1e-3)
+        tr_slots = [s for ss in tr for s in ss]
+        te_slots = [s for ss in te for s in ss]
+        _flush(f"    samples: train {len(tr_slots)} held-out {len(te_slots)}")
+        base = np.mean([sur.baseline_loss(te_slots[i:i + 16]) for i in range(0, len(te_slots), 16)])
+        before = np.mean([sur.loss(te_slots[i:i + 16]) for i in range(0, len(te_slots), 16)])
+        fit(sur, tr_slots, steps=800, batch=16, lr=1e-3, log_every=100, log=_flush)
+        after = np.mean([sur.loss(te_slots[i:i + 16]) for i in range(0, len(te_slots), 16)])
+        _flush(f"    held-out loss: ballistic {base:.4f}  untrained {before:.4f}  trained {after:.4f}")
+        assert after < 0.6 * base
+
+        errs = {ADVANCE_NETWORK: [], ADVANCE_BALLISTIC: []}
+        for Pq, ps in test_scenes:
+            ref = rollout(None, Pq, ps, nsub, k, ADVANCE_PLAIN)["x"]
+            for mode in errs:
+                r = rollout(sur, Pq, ps, nsub, k, mode, tol=1e-3)
+                assert r["nonfinite"] == 0
+                errs[mode].append(float(np.linalg.norm(r["x"] - ref, axis=1).mean()))
+        net, bal = np.mean(errs[ADVANCE_NETWORK]), np.mean(errs[ADVANCE_BALLISTIC])
+        _flush(f"    mean position error after {nsub} substeps: network {net:.3e} m  ballistic {bal:.3e} m")
+        assert net < bal
