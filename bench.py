@@ -89,6 +89,106 @@ def copy_calibration(gib: float = 2.0) -> None:
     print(json.dumps({"copy_bytes_each_way": nbytes, "copies": 3}), flush=True)
 
 
+def learned(args) -> dict:
+    """Learned-vs-plain: train the grid U-Net on trajectories of colliding 2D scenes
+    (256^2 grid, ~60k particles each), then roll held-out scenes out for
+    `--rollout` substeps in macro steps of 8 with the network (plain substeps where
+    the residual test fails) and with the plain solver alone."""
+    from mpm_snow_sim2d_amd.params import Options, snow_params
+    from mpm_snow_sim2d_amd.scenes import colliding_scene
+    from mpm_snow_sim2d_amd.surrogate import (ADVANCE_BALLISTIC, ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate, fit,
+                                              generate, rollout)
+
+    cells, dt, k, npart = 256, 5e-5, 8, 60_000
+    P = snow_params(2, cells, dt)
+    ntr, nte = args.train_scenes, args.test_scenes
+    per = args.gen_substeps // k
+    t0 = time.perf_counter()
+    train = [(P, colliding_scene(1000 + i, npart)) for i in range(ntr)]
+    test = [(P, colliding_scene(5000 + i, npart)) for i in range(nte)]
+    log(f"[learned] scenes ready ({time.perf_counter() - t0:.1f} s)")
+    sur = Surrogate(cells, cells, max_batch=args.batch, capacity=(ntr + nte) * per, seed=0, d_ref=0.1)
+    t0 = time.perf_counter()
+    tr = generate(sur, train, args.gen_substeps, k, wall_tol=args.tol)
+    te = generate(sur, test, args.gen_substeps, k, first_slot=ntr * per, wall_tol=args.tol)
+    gen_s = time.perf_counter() - t0
+    tr_slots = [x for ss in tr for x in ss]
+    te_slots = [x for ss in te for x in ss]
+    log(f"[learned] {len(tr_slots)} training / {len(te_slots)} held-out samples in {gen_s:.1f} s")
+
+    def mean_loss(fn, slots):
+        return float(np.mean([fn(slots[i:i + args.batch]) for i in range(0, len(slots), args.batch)]))
+
+    base_loss = mean_loss(sur.baseline_loss, te_slots)
+    t0 = time.perf_counter()
+    fit(sur, tr_slots, steps=args.train_steps, batch=args.batch, lr=args.lr, log_every=max(args.train_steps // 10, 1),
+        log=log)
+    train_s = time.perf_counter() - t0
+    val_loss = mean_loss(sur.loss, te_slots)
+    log(f"[learned] trained {args.train_steps} steps in {train_s:.1f} s; held-out loss {val_loss:.4f} "
+        f"(ballistic {base_loss:.4f})")
+
+    fwd_ms = sur.time(1, iters=20)
+    step_ms = sur.time(args.batch, iters=5, train=True)
+    flops = sur.flops(1)
+
+    res = {ADVANCE_PLAIN: [], ADVANCE_NETWORK: [], ADVANCE_BALLISTIC: []}
+    for i, (Pq, ps) in enumerate(test):
+        ref = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN)
+        res[ADVANCE_PLAIN].append({"wall": ref["wall"], "err": 0.0, "counts": ref["counts"]})
+        for mode in (ADVANCE_NETWORK, ADVANCE_BALLISTIC):
+            r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol)
+            err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean())
+            res[mode].append({"wall": r["wall"], "err": err, "counts": r["counts"], "bad": r["nonfinite"]})
+        log(f"[learned] scene {i}: plain {ref['wall'] * 1e3:.1f} ms; network {res[ADVANCE_NETWORK][-1]['wall'] * 1e3:.1f} ms "
+            f"err {res[ADVANCE_NETWORK][-1]['err']:.2e} m uses {res[ADVANCE_NETWORK][-1]['counts']}; ballistic err "
+            f"{res[ADVANCE_BALLISTIC][-1]['err']:.2e} m")
+    sur.close()
+
+    def agg(mode, key):
+        return float(np.mean([r[key] for r in res[mode]]))
+
+    t_plain, t_net = agg(ADVANCE_PLAIN, "wall"), agg(ADVANCE_NETWORK, "wall")
+    net_steps = sum(r["counts"][ADVANCE_NETWORK] for r in res[ADVANCE_NETWORK])
+    all_steps = sum(sum(r["counts"].values()) for r in res[ADVANCE_NETWORK])
+    err_net = agg(ADVANCE_NETWORK, "err")
+    achieved = flops / (fwd_ms * 1e-3) / 1e12
+    return {
+        "metric": METRIC,
+        "value": round(t_plain / t_net, 4),
+        "unit": "x (plain wall time / learned wall time, same rollout)",
+        "n_gpus": 1,
+        "steps": args.rollout,
+        "warmup": 0,
+        "ms_per_step": round(t_net / args.rollout * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (network, FP32 accumulation); f32 (solver)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"learned 2D surrogate: 256^2 grid, colliding scenes of ~{npart} particles (2-3 disks/boxes), "
+                        f"4-level BF16 U-Net 32-64-128-128, k=8 substeps per macro step, residual tolerance {args.tol}",
+            "train_scenes": ntr, "generated_substeps_per_scene": args.gen_substeps,
+            "train_samples": len(tr_slots), "train_steps": args.train_steps, "batch": args.batch,
+            "generate_s": round(gen_s, 2), "train_s": round(train_s, 2),
+            "heldout_loss": round(val_loss, 5), "heldout_loss_ballistic": round(base_loss, 5),
+            "test_scenes": nte, "rollout_substeps": args.rollout,
+            "plain_ms": round(t_plain * 1e3, 3), "learned_ms": round(t_net * 1e3, 3),
+            "mean_pos_err_m": err_net, "mean_pos_err_pct_domain": round(100 * err_net, 4),
+            "within_1pct": bool(err_net <= 0.01),
+            "ballistic_pos_err_m": agg(ADVANCE_BALLISTIC, "err"),
+            "network_macro_steps_share": round(net_steps / max(all_steps, 1), 4),
+            "forward_ms": round(fwd_ms, 4), "train_step_ms": round(step_ms, 4),
+        },
+        "roofline": {"bound": "mfma", "kernel": "U-Net forward (BF16 3x3 convolutions, 1 image 256^2)",
+                     "achieved": round(achieved, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(achieved / 2500.0, 5), "traffic": None,
+                     "flops_per_forward": flops},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +196,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--sort-every", type=int, default=16)
     ap.add_argument("--n-side", type=int, default=126)
-    ap.add_argument("--workload", choices=["cube", "large", "batch", "snowballs2d"], default="cube")
+    ap.add_argument("--workload", choices=["cube", "large", "batch", "snowballs2d", "learned"], default="cube")
+    ap.add_argument("--train-scenes", type=int, default=32)
+    ap.add_argument("--test-scenes", type=int, default=4)
+    ap.add_argument("--gen-substeps", type=int, default=800)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--rollout", type=int, default=2000)
+    ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--tile", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -111,6 +219,17 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (one process on one "
               "MI355X; the multi-GPU domain decomposition is design-only)", file=sys.stderr)
         return 2
+
+    if args.workload == "learned":
+        out = learned(args)
+        if not args.no_cpu_baseline:
+            from mpm_snow_sim2d_amd.params import snow_params
+            from mpm_snow_sim2d_amd.scenes import colliding_scene
+            log("[bench] CPU baseline ...")
+            out["cpu_baseline"] = cpu_baseline(snow_params(2, 256, 5e-5), colliding_scene(5000, 60_000),
+                                               args.cpu_substeps, "a held-out scene, plain solver")
+        print(json.dumps(out), flush=True)
+        return 0
 
     from mpm_snow_sim2d_amd import _lib
     from mpm_snow_sim2d_amd.params import Options, snow_params

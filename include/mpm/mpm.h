@@ -231,6 +231,10 @@ int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, floa
 int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,
                           float* residual);
 
+/* Average device time (ms) of `iters` network passes on nb images of the batch
+ * buffers: forward only (train = 0) or forward + loss + backward (train = 1). */
+int mpm_surrogate_time(mpm_surrogate* g, int nb, int iters, int train, double* ms);
+
 /* Stable sort of n 32-bit keys on the device (the binning sort on its own, for
  * callers with their own keys).  Host arrays; out_perm[i] = index of the i-th
  * smallest key, ties in input order. key_bits <= 32. */

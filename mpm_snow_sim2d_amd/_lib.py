@@ -35,7 +35,7 @@ EXPORTS = [
     "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
     "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_forward", "mpm_surrogate_gradient",
     "mpm_surrogate_record", "mpm_surrogate_get_sample", "mpm_surrogate_train", "mpm_surrogate_baseline_loss",
-    "mpm_surrogate_advance",
+    "mpm_surrogate_advance", "mpm_surrogate_time",
 ]
 
 
@@ -116,6 +116,7 @@ def load() -> ctypes.CDLL:
         "mpm_surrogate_train": (i32, [vp, vp, i32, ctypes.c_float, vp]),
         "mpm_surrogate_baseline_loss": (i32, [vp, vp, i32, vp]),
         "mpm_surrogate_advance": (i32, [vp, vp, i32, i32, ctypes.c_float, vp, vp]),
+        "mpm_surrogate_time": (i32, [vp, i32, i32, i32, P(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -70,17 +71,32 @@ __device__ inline void block_add(float v, float* dst, float* sh)
 }
 
 // Scatter of the features (mode 0) or of the target momentum at the saved
-// positions (mode 1) of the particles in slots [first, first + count).
+// positions (mode 1) of the particles in slots [first, first + count).  The
+// particles of a workgroup are spatially sorted (the solver bins them), so their
+// stencils cover a small box of nodes: the workgroup accumulates that box in LDS
+// and adds it to the image once per node; a workgroup whose box does not fit
+// (after particles have mixed) adds its contributions directly.
+constexpr int CAP_NODES = 1536;   // LDS box capacity in nodes (5 channels, 30 KB)
+
 __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int count, Phys ph, int h, int w,
                                                 float band, float* __restrict__ sample, float* __restrict__ xs,
                                                 SlotInfo* __restrict__ info, int mode)
 {
     __shared__ float sh[4];
+    __shared__ float tile[5 * CAP_NODES];
+    __shared__ int box[4];
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int n = P.n;
     const size_t hw = (size_t)h * w;
+    const int nch = mode == 0 ? 5 : 2;
     float pm = 0.f, inband = 0.f;
-    if (k < count) {
+    float val[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    Stencil<2, 2> st;
+    const bool live = k < count;
+    if (threadIdx.x == 0) {
+        box[0] = INT_MAX; box[1] = INT_MAX; box[2] = INT_MIN; box[3] = INT_MIN;
+    }
+    if (live) {
         const int p = first + k;
         const uint32_t id = P.pid[p];
         float x0 = P.x[p], x1 = P.x[n + p];
@@ -95,33 +111,66 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         pm = m;
         const float xg[2] = {x0 * ph.inv_dx, x1 * ph.inv_dx};
         if (xg[0] < band || xg[0] > (float)ph.W - band || xg[1] < band || xg[1] > (float)ph.H - band) inband = m;
-        Stencil<2, 2> st;
         st.init(xg);
         const float v0 = P.v[p], v1 = P.v[n + p];
-        float jp = 0.f, ls = 0.f;
         if (mode == 0) {
-            jp = P.Jp[p];
             const float F[4] = {P.F[p], P.F[n + p], P.F[2 * n + p], P.F[3 * n + p]};
-            ls = 0.5f * logf(fmaxf(fabsf(det<2>(F)), 1e-30f));
+            val[0] = m;
+            val[1] = m * v0;
+            val[2] = m * v1;
+            val[3] = m * P.Jp[p];
+            val[4] = m * 0.5f * logf(fmaxf(fabsf(det<2>(F)), 1e-30f));
+        } else {
+            val[0] = m * v0;
+            val[1] = m * v1;
         }
+    }
+    __syncthreads();
+    if (live) {
+        atomicMin(&box[0], st.base[0]);
+        atomicMin(&box[1], st.base[1]);
+        atomicMax(&box[2], st.base[0]);
+        atomicMax(&box[3], st.base[1]);
+    }
+    __syncthreads();
+    const bool any_live = box[2] >= box[0];
+    const int bx0 = box[0], by0 = box[1];
+    const int bw = any_live ? box[2] - bx0 + 3 : 0, bh = any_live ? box[3] - by0 + 3 : 0;
+    const size_t ch0 = mode == 0 ? 0 : 5;
+    if (any_live && bw <= CAP_NODES && bh <= CAP_NODES && bw * bh <= CAP_NODES) {
+        const int nn = bw * bh;
+        for (int q = threadIdx.x; q < nn * nch; q += 256) tile[q] = 0.f;
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const float wt = st.w[0][a] * st.w[1][b];
+                    const int q = (st.base[1] + b - by0) * bw + (st.base[0] + a - bx0);
+                    for (int c = 0; c < nch; ++c) atomicAdd(&tile[c * nn + q], wt * val[c]);
+                }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nn; q += 256) {
+            const int i = bx0 + q % bw, j = by0 + q / bw;
+            if (i < 0 || i >= w || j < 0 || j >= h) continue;
+            bool any = false;
+            for (int c = 0; c < nch; ++c) any |= tile[c * nn + q] != 0.f;
+            if (!any) continue;
+            const size_t g = (size_t)j * w + i;
+            for (int c = 0; c < nch; ++c) atomicAdd(sample + (ch0 + c) * hw + g, tile[c * nn + q]);
+        }
+    } else if (live) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
                 const int i = st.base[0] + a, j = st.base[1] + b;
                 if (i < 0 || i >= w || j < 0 || j >= h) continue;
-                const float wm = st.w[0][a] * st.w[1][b] * m;
-                const size_t q = (size_t)j * w + i;
-                if (mode == 0) {
-                    atomicAdd(sample + q, wm);
-                    atomicAdd(sample + hw + q, wm * v0);
-                    atomicAdd(sample + 2 * hw + q, wm * v1);
-                    atomicAdd(sample + 3 * hw + q, wm * jp);
-                    atomicAdd(sample + 4 * hw + q, wm * ls);
-                } else {
-                    atomicAdd(sample + 5 * hw + q, wm * v0);
-                    atomicAdd(sample + 6 * hw + q, wm * v1);
-                }
+                const float wt = st.w[0][a] * st.w[1][b];
+                const size_t g = (size_t)j * w + i;
+                for (int c = 0; c < nch; ++c) atomicAdd(sample + (ch0 + c) * hw + g, wt * val[c]);
             }
     }
     if (mode == 0) {
@@ -400,6 +449,9 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
         hipMalloc(&g->loss_d, 256) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "surrogate allocation failed"));
     if (hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
+        hipMemsetAsync(g->in, 0, (size_t)max_batch * hw * 32 * sizeof(bf16), g->st) != hipSuccess ||
+        hipMemsetAsync(g->wm, 0, (size_t)max_batch * hw * sizeof(float), g->st) != hipSuccess ||
+        hipMemsetAsync(g->target, 0, (size_t)max_batch * hw * 2 * sizeof(float), g->st) != hipSuccess ||
         hipStreamSynchronize(g->st) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "surrogate initialisation failed"));
     *out = g;
@@ -552,6 +604,31 @@ int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, floa
     for (size_t i = 0; i < wm.size(); ++i) l += (double)wm[i] * ((double)t[2 * i] * t[2 * i] + (double)t[2 * i + 1] * t[2 * i + 1]);
     *loss = (float)(l / nb);
     return MPM_OK;
+}
+
+int mpm_surrogate_time(mpm_surrogate* g, int nb, int iters, int train, double* ms)
+{
+    if (!g || !ms) return sfail(MPM_E_INVALID, "NULL argument");
+    if (nb < 1 || nb > g->max_batch || iters < 1) return sfail(MPM_E_INVALID, "nb/iters out of range");
+    hipEvent_t a = nullptr, b = nullptr;
+    S_TRY(hipEventCreate(&a));
+    S_TRY(hipEventCreate(&b));
+    // whatever the batch buffers hold: the kernels' cost does not depend on values
+    int r = MPM_OK;
+    float t = 0.f;
+    if (hipEventRecord(a, g->st) != hipSuccess) r = sfail(MPM_E_HIP, "hipEventRecord failed");
+    for (int i = 0; i < iters && r == MPM_OK; ++i) {
+        hipError_t e = train ? g->net.loss_and_backward(nb, g->in, g->wm, g->target, g->loss_d, true)
+                             : g->net.forward(nb, g->in, g->wm);
+        if (e != hipSuccess) r = sfail(MPM_E_HIP, std::string("network: ") + hipGetErrorString(e));
+    }
+    if (r == MPM_OK && (hipEventRecord(b, g->st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                        hipEventElapsedTime(&t, a, b) != hipSuccess))
+        r = sfail(MPM_E_HIP, "event timing failed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (r == MPM_OK) *ms = (double)t / iters;
+    return r;
 }
 
 int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,

@@ -73,14 +73,14 @@ private:
     unet::bf16* wf_[NLAYERS] = {};
     unet::bf16* wb_[NLAYERS] = {};
     // activations (forward) and their gradients (backward), BF16 NHWC
-    unet::bf16 *z0_, *a0_, *p0_, *z1_, *a1_, *p1_, *z2_, *a2_, *p2_, *z3_, *a3_;
-    unet::bf16 *u2_, *zd2_, *b2_, *u1_, *zd1_, *b1_, *u0_, *zd0_, *b0_;
+    unet::bf16 *z0_, *a0_, *z1_, *a1_, *z2_, *a2_, *z3_, *a3_;
+    unet::bf16 *zd2_, *b2_, *zd1_, *b1_, *zd0_, *b0_;
     unet::bf16 *g_b0_, *g_u0_, *g_a0_, *g_b1_, *g_u1_, *g_a1_, *g_b2_, *g_u2_, *g_a2_, *g_a3_, *g_p2_, *g_p1_,
         *g_p0_;
     float* out_ = nullptr;
     float* dproj_ = nullptr;
-    float* sums_ = nullptr;    // [max_batch][2] projection sums
-    float* sums2_ = nullptr;   // [max_batch][2] gradient sums
+    float* part_ = nullptr;    // [max_batch][64][2] partial projection sums
+    float* part2_ = nullptr;   // [max_batch][64][2] partial gradient sums
     std::vector<void*> allocs_;
     void* alloc(size_t bytes, hipError_t* e);
 };

@@ -7,6 +7,8 @@
 //   A (16 x 32, rows i, reduction k):  lane holds A[r][8g .. 8g+7]
 //   B (32 x 16, reduction k, cols j):  lane holds B[8g .. 8g+7][r]
 //   D (16 x 16, FP32):                 lane holds D[4g .. 4g+3][r]
+// The convolutions put output channels on the rows and pixels on the columns, so
+// that a lane's four results are consecutive channels of one NHWC pixel.
 // A and B place the same reduction index in the same (lane group, element) slot,
 // so the product does not depend on how the hardware orders the reduction.
 #include <hip/hip_runtime.h>
@@ -41,28 +43,64 @@ __device__ __forceinline__ bf16x8 zero8()
     return v;
 }
 
-constexpr int CONV_MT = 2;                 // 16-pixel row tiles per wave
-constexpr int CONV_PIX = 4 * 16 * CONV_MT; // pixels per workgroup
+// Eight channels [c, c+8) of pixel (b, y, x) of an input as a layer at resolution
+// h x w sees it: stored at that resolution (mode 0), at half resolution and
+// upsampled by nearest copy (mode 1), or at double resolution and 2x2-average
+// pooled (mode 2, summed in FP32 in raster order and rounded once).
+__device__ __forceinline__ bf16x8 load_in(const bf16* x, int cs, int c, int b, int y, int xx, int h, int w,
+                                          int mode)
+{
+    if (mode == 0) return *reinterpret_cast<const bf16x8*>(x + ((size_t)(b * h + y) * w + xx) * cs + c);
+    if (mode == 1) {
+        const int hh = h >> 1, ww = w >> 1;
+        return *reinterpret_cast<const bf16x8*>(x + ((size_t)(b * hh + (y >> 1)) * ww + (xx >> 1)) * cs + c);
+    }
+    const int h2 = 2 * h, w2 = 2 * w;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(
+                x + ((size_t)(b * h2 + 2 * y + a) * w2 + 2 * xx + bb) * cs + c);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+        }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = (bf16)(0.25f * s[j]);
+    return o;
+}
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution as implicit GEMM: rows = pixels, columns = output channels,
-// reduction = (tap, input channel).  Workgroup: 4 waves x 32 pixels, 16*NT output
-// channels (blockIdx.y selects the channel tile).
-template <int NT, bool SPLIT>
-__global__ void __launch_bounds__(256) k_conv3x3(ConvArgs a)
+// reduction = (tap, 32-channel chunk).  Each wave owns a tile of 16*MT pixels x
+// 16*NT output channels; a workgroup of 4 waves holds 4/KS tiles and splits each
+// tile's reduction KS ways (waves s = 0..KS-1 take chunks s, s+KS, ...), adding the
+// partial sums through LDS before the epilogue.  Small (low-resolution) layers use
+// KS > 1 so that they still fill the chip.
+template <int MT, int NT, int KS, bool SPLIT>
+__global__ void __launch_bounds__(256) k_conv3x3(ConvArgs a, int ntiles_co)
 {
+    constexpr int TPW = 4 / KS;                     // tiles per workgroup
+    constexpr int NACC = MT * NT * 4;               // accumulator floats per lane
+    __shared__ float red[(KS > 1 ? (KS - 1) * TPW : 1) * NACC * 64];
+
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
+    const int ks = wave % KS, tloc = wave / KS;
+    const int tile = blockIdx.x * TPW + tloc;
+    const int co_t = tile % ntiles_co, px_t = tile / ntiles_co;
     const int cin = a.c1 + a.c2;
     const int hw = a.h * a.w;
     const int npix = a.nb * hw;
-    const int co0 = blockIdx.y * 16 * NT;
-    const int pbase = blockIdx.x * CONV_PIX + wave * 16 * CONV_MT;
+    const int co0 = co_t * 16 * NT;
+    const int pbase = px_t * 16 * MT;
 
-    int pb[CONV_MT], py[CONV_MT], px[CONV_MT];
-    bool pv[CONV_MT];
+    int pb[MT], py[MT], px[MT];
+    bool pv[MT];
 #pragma unroll
-    for (int mt = 0; mt < CONV_MT; ++mt) {
+    for (int mt = 0; mt < MT; ++mt) {
         const int p = pbase + mt * 16 + r;
         pv[mt] = p < npix;
         const int pp = pv[mt] ? p : 0;
@@ -72,82 +110,168 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs a)
         px[mt] = rem - py[mt] * a.w;
     }
 
-    f32x4 acc[CONV_MT][NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int mt = 0; mt < CONV_MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int tap = 0; tap < 9; ++tap) {
+    const int nchunk = cin >> 5;
+    const int nk = 9 * nchunk;
+    const bool tile_live = pbase < npix;
+    // fragments of reduction step kk: A (MT pixel tiles) and B (NT channel tiles)
+    auto load = [&](int kk, bf16x8* af, bf16x8* bfr) {
+        const int tap = kk / nchunk;
+        const int ci0 = (kk - tap * nchunk) << 5;
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        int q[CONV_MT];
-        bool ok[CONV_MT];
+        const bool first = ci0 < a.c1;
+        const bf16* xs = first ? a.x1 : a.x2;
+        const int cs = first ? a.c1 : a.c2;
+        const int cc = (first ? ci0 : ci0 - a.c1) + 8 * g;
+        const int md = first ? a.mode1 : 0;
 #pragma unroll
-        for (int mt = 0; mt < CONV_MT; ++mt) {
+        for (int mt = 0; mt < MT; ++mt) {
             const int yy = py[mt] + dy, xx = px[mt] + dx;
-            ok[mt] = pv[mt] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
-            q[mt] = (pb[mt] * a.h + yy) * a.w + xx;
+            const bool ok = pv[mt] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+            af[mt] = ok ? load_in(xs, cs, cc, pb[mt], yy, xx, a.h, a.w, md) : zero8();
         }
-        const bf16* wrow = a.wt + ((size_t)tap * a.cout + co0 + r) * cin + 8 * g;
-        for (int ci0 = 0; ci0 < cin; ci0 += 32) {
-            const bool first = ci0 < a.c1;
-            const bf16* xs = first ? a.x1 : a.x2;
-            const int cs = first ? a.c1 : a.c2;
-            const int cc = (first ? ci0 : ci0 - a.c1) + 8 * g;
-            bf16x8 af[CONV_MT];
+        const bf16* wrow = a.wt + ((size_t)tap * a.cout + co0 + r) * cin + ci0 + 8 * g;
 #pragma unroll
-            for (int mt = 0; mt < CONV_MT; ++mt)
-                af[mt] = ok[mt] ? *reinterpret_cast<const bf16x8*>(xs + (size_t)q[mt] * cs + cc) : zero8();
+        for (int t = 0; t < NT; ++t) bfr[t] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * t * cin);
+    };
+    if (tile_live && ks < nk) {
+        // a ring of DEPTH reduction steps in registers: the loads of step i + DEPTH
+        // are issued as soon as step i has gone to the matrix cores, so DEPTH load
+        // round trips overlap
+        constexpr int DEPTH = 4;
+        const int nsteps = (nk - ks + KS - 1) / KS;
+        bf16x8 fa[DEPTH][MT], fb[DEPTH][NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * t * cin + ci0);
+        for (int j = 0; j < DEPTH; ++j)
+            if (j < nsteps) load(ks + j * KS, fa[j], fb[j]);
+        for (int i = 0; i < nsteps; i += DEPTH) {
 #pragma unroll
-                for (int mt = 0; mt < CONV_MT; ++mt)
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mt], bfr, acc[mt][t], 0, 0, 0);
+            for (int j = 0; j < DEPTH; ++j) {
+                if (i + j < nsteps) {
+                    // weights as the A operand (rows = output channels), pixels as B:
+                    // each lane ends up with 4 consecutive channels of one pixel
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+                            acc[mt][t] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j][t], fa[j][mt], acc[mt][t], 0, 0, 0);
+                    if (i + j + DEPTH < nsteps) load(ks + (i + j + DEPTH) * KS, fa[j], fb[j]);
+                }
             }
         }
     }
 
+    if constexpr (KS > 1) {
+        if (ks > 0) {
+            float* dst = red + ((tloc * (KS - 1) + ks - 1) * NACC) * 64 + lane;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = co0 + 16 * t + r;
-        const float bias = (!SPLIT && a.bias) ? a.bias[co] : 0.f;
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int mt = 0; mt < CONV_MT; ++mt)
+                for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int p = pbase + mt * 16 + 4 * g + e;
-                if (p >= npix) continue;
-                const float v = acc[mt][t][e] + bias;
-                if constexpr (!SPLIT) {
-                    if (a.z) a.z[(size_t)p * a.cout + co] = (bf16)v;
-                    if (a.a) a.a[(size_t)p * a.cout + co] = (bf16)gelu(v);
-                } else {
-                    if (co < a.split) {
-                        bf16* d = a.o1 + (size_t)p * a.split + co;
-                        *d = (bf16)(a.add1 ? (float)*d + v : v);
-                    } else {
-                        const int c2n = a.cout - a.split;
-                        bf16* d = a.o2 + (size_t)p * c2n + (co - a.split);
-                        *d = (bf16)(a.add2 ? (float)*d + v : v);
-                    }
+                    for (int e = 0; e < 4; ++e) dst[((mt * NT + t) * 4 + e) * 64] = acc[mt][t][e];
+        }
+        __syncthreads();
+        if (ks > 0) return;
+#pragma unroll
+        for (int j = 0; j < KS - 1; ++j) {
+            const float* src = red + ((tloc * (KS - 1) + j) * NACC) * 64 + lane;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[mt][t][e] += src[((mt * NT + t) * 4 + e) * 64];
+        }
+    }
+    if (!tile_live) return;
+
+    // D[4g+e][r] of tile (mt, t): output channel co0 + 16t + 4g + e of pixel
+    // pbase + 16mt + r; four channels go out in one 8-byte store
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int p = pbase + mt * 16 + r;
+        if (p >= npix) continue;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int cb = co0 + 16 * t + 4 * g;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = acc[mt][t][e] + ((!SPLIT && a.bias) ? a.bias[cb + e] : 0.f);
+            if constexpr (!SPLIT) {
+                bf16x4 zo, ao;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    zo[e] = (bf16)v[e];
+                    ao[e] = (bf16)gelu(v[e]);
                 }
+                if (a.z) *reinterpret_cast<bf16x4*>(a.z + (size_t)p * a.cout + cb) = zo;
+                if (a.a) *reinterpret_cast<bf16x4*>(a.a + (size_t)p * a.cout + cb) = ao;
+            } else {
+                // split is a multiple of 32: the four channels lie on one side
+                const bool lo = cb < a.split;
+                const int stride = lo ? a.split : a.cout - a.split;
+                bf16* d = lo ? a.o1 + (size_t)p * stride + cb : a.o2 + (size_t)p * stride + (cb - a.split);
+                const int add = lo ? a.add1 : a.add2;
+                bf16x4 o;
+                if (add) {
+                    const bf16x4 old = *reinterpret_cast<const bf16x4*>(d);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) o[e] = (bf16)((float)old[e] + v[e]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) o[e] = (bf16)v[e];
+                }
+                *reinterpret_cast<bf16x4*>(d) = o;
             }
+        }
     }
 }
 
+template <int MT, int NT, int KS, bool SPLIT>
+hipError_t launch_conv_cfg(const ConvArgs& a, int npix, hipStream_t st)
+{
+    const int ntiles_co = a.cout / (16 * NT);
+    const int ntiles = ((npix + 16 * MT - 1) / (16 * MT)) * ntiles_co;
+    constexpr int TPW = 4 / KS;
+    k_conv3x3<MT, NT, KS, SPLIT><<<(ntiles + TPW - 1) / TPW, 256, 0, st>>>(a, ntiles_co);
+    return hipGetLastError();
+}
+
+// Tile shape and reduction split by layer shape: 16*NT output channels per wave
+// (NT = 4 when the channel count allows, else 2), the largest of 64/32/16 pixels
+// per wave that still gives >= 1024 waves, and the reduction split until about
+// 2048 waves run.
 template <bool SPLIT>
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st)
 {
     const int npix = a.nb * a.h * a.w;
     if (npix == 0) return hipSuccess;
-    const int tiles = a.cout / 16;
-    const int nt = tiles % 8 == 0 ? 8 : tiles % 4 == 0 ? 4 : 2;
-    dim3 grid((npix + CONV_PIX - 1) / CONV_PIX, tiles / nt), blk(256);
-    if (nt == 8) k_conv3x3<8, SPLIT><<<grid, blk, 0, st>>>(a);
-    else if (nt == 4) k_conv3x3<4, SPLIT><<<grid, blk, 0, st>>>(a);
-    else k_conv3x3<2, SPLIT><<<grid, blk, 0, st>>>(a);
-    return hipGetLastError();
+    const int nt = (a.cout % 64 == 0) ? 4 : 2;
+    const int co_tiles = a.cout / (16 * nt);
+    auto waves_at = [&](int m) { return ((npix + 16 * m - 1) / (16 * m)) * co_tiles; };
+    const int mt = waves_at(4) >= 1024 ? 4 : waves_at(2) >= 1024 ? 2 : 1;
+    const int tiles = waves_at(mt);
+    const int chunks = 9 * ((a.c1 + a.c2) / 32);
+    int ks = 1;
+    while (ks < 4 && tiles * ks < 2048 && chunks >= 4 * ks) ks *= 2;
+#define CONV_CASE(M, N, K)                                        \
+    if (mt == M && nt == N && ks == K) return launch_conv_cfg<M, N, K, SPLIT>(a, npix, st);
+    CONV_CASE(1, 2, 1) CONV_CASE(1, 2, 2) CONV_CASE(1, 2, 4)
+    CONV_CASE(1, 4, 1) CONV_CASE(1, 4, 2) CONV_CASE(1, 4, 4)
+    CONV_CASE(2, 2, 1) CONV_CASE(2, 2, 2) CONV_CASE(2, 2, 4)
+    CONV_CASE(2, 4, 1) CONV_CASE(2, 4, 2) CONV_CASE(2, 4, 4)
+    CONV_CASE(4, 2, 1) CONV_CASE(4, 2, 2) CONV_CASE(4, 2, 4)
+    CONV_CASE(4, 4, 1) CONV_CASE(4, 4, 2) CONV_CASE(4, 4, 4)
+#undef CONV_CASE
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -160,8 +284,8 @@ constexpr int WG_PIX = 2048;   // pixels per workgroup (split of the reduction)
 constexpr int WG_LDS = 40;     // LDS row stride (bf16) of a transposed slab
 
 __global__ void __launch_bounds__(256) k_conv3x3_wgrad(const bf16* __restrict__ x1, const bf16* __restrict__ x2,
-                                                      int c1, int c2, const bf16* __restrict__ dz, int cout,
-                                                      int nb, int h, int w, float* __restrict__ dw)
+                                                      int c1, int c2, int mode1, const bf16* __restrict__ dz,
+                                                      int cout, int nb, int h, int w, float* __restrict__ dw)
 {
     __shared__ __attribute__((aligned(16))) bf16 tz[4][32 * WG_LDS];
     __shared__ __attribute__((aligned(16))) bf16 tx[4][32 * WG_LDS];
@@ -201,9 +325,9 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(const bf16* __restrict__ 
             const int b = p / hw, rem = p - b * hw;
             const int yy = rem / w + dy, xx = rem % w + dx;
             if (yy >= 0 && yy < h && xx >= 0 && xx < w) {
-                const bf16* xr = xs + (size_t)((b * h + yy) * w + xx) * cs + cc + sc;
-                v0 = *reinterpret_cast<const bf16x8*>(xr);
-                v1 = *reinterpret_cast<const bf16x8*>(xr + 8);
+                const int md = first ? mode1 : 0;
+                v0 = load_in(xs, cs, cc + sc, b, yy, xx, h, w, md);
+                v1 = load_in(xs, cs, cc + sc + 8, b, yy, xx, h, w, md);
             }
         }
         __syncthreads();
@@ -402,10 +526,14 @@ __device__ __forceinline__ float block_sum(float v, float* sh)
     return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// Per-block partial sums part[image][block][2] of wm_i out_i (no atomics: the
+// projection kernel adds the partials of its image itself).
+constexpr int HEAD_BLOCKS = 64;
+
 __global__ void __launch_bounds__(256) k_head_forward(const bf16* __restrict__ a, int nb, int hw,
                                                      const float* __restrict__ wh, const float* __restrict__ bh,
                                                      const float* __restrict__ wm, float* __restrict__ out,
-                                                     float* __restrict__ sums)
+                                                     float* __restrict__ part)
 {
     __shared__ float sh[4];
     // blockIdx.y = image; pixels of the image grid-strided over blockIdx.x
@@ -432,20 +560,26 @@ __global__ void __launch_bounds__(256) k_head_forward(const bf16* __restrict__ a
     const float t0 = block_sum(s0, sh);
     const float t1 = block_sum(s1, sh);
     if (threadIdx.x == 0) {
-        atomicAdd(sums + 2 * b, t0);
-        atomicAdd(sums + 2 * b + 1, t1);
+        part[((size_t)b * HEAD_BLOCKS + blockIdx.x) * 2] = t0;
+        part[((size_t)b * HEAD_BLOCKS + blockIdx.x) * 2 + 1] = t1;
     }
 }
 
-// out' = out - sums_b in place; loss and dL/dout'
+// out' = out - (sum of the image's partials) in place; with a target, the loss and
+// dL/dout' with its per-block partial sums part2 [image][block][2].
 __global__ void __launch_bounds__(256) k_head_loss(float* __restrict__ out, const float* __restrict__ target,
                                                   const float* __restrict__ wm, int nb, int hw,
-                                                  const float* __restrict__ sums, float* __restrict__ loss,
-                                                  float* __restrict__ dproj, float* __restrict__ sums2)
+                                                  const float* __restrict__ part, int nparts,
+                                                  float* __restrict__ loss, float* __restrict__ dproj,
+                                                  float* __restrict__ part2)
 {
     __shared__ float sh[4];
     const int b = blockIdx.y;
-    const float m0 = sums[2 * b], m1 = sums[2 * b + 1];
+    float m0 = 0.f, m1 = 0.f;
+    for (int j = 0; j < nparts; ++j) {
+        m0 += part[((size_t)b * HEAD_BLOCKS + j) * 2];
+        m1 += part[((size_t)b * HEAD_BLOCKS + j) * 2 + 1];
+    }
     const float inv_nb = 1.0f / (float)nb;
     float l = 0.f, g0 = 0.f, g1 = 0.f;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += gridDim.x * 256) {
@@ -473,21 +607,25 @@ __global__ void __launch_bounds__(256) k_head_loss(float* __restrict__ out, cons
     if (threadIdx.x == 0) {
         atomicAdd(loss, tl * inv_nb);
         if (dproj) {
-            atomicAdd(sums2 + 2 * b, t0);
-            atomicAdd(sums2 + 2 * b + 1, t1);
+            part2[((size_t)b * HEAD_BLOCKS + blockIdx.x) * 2] = t0;
+            part2[((size_t)b * HEAD_BLOCKS + blockIdx.x) * 2 + 1] = t1;
         }
     }
 }
 
 __global__ void __launch_bounds__(256) k_head_backward(const bf16* __restrict__ a, const float* __restrict__ dproj,
-                                                      const float* __restrict__ sums2,
+                                                      const float* __restrict__ part2, int nparts,
                                                       const float* __restrict__ wm, int nb, int hw,
                                                       const float* __restrict__ wh, bf16* __restrict__ da,
                                                       float* __restrict__ dwh, float* __restrict__ dbh)
 {
     __shared__ float sh[4];
     const int b = blockIdx.y;
-    const float t0 = sums2[2 * b], t1 = sums2[2 * b + 1];
+    float t0 = 0.f, t1 = 0.f;
+    for (int j = 0; j < nparts; ++j) {
+        t0 += part2[((size_t)b * HEAD_BLOCKS + j) * 2];
+        t1 += part2[((size_t)b * HEAD_BLOCKS + j) * 2 + 1];
+    }
     float gw[2 * HEAD_C];
 #pragma unroll
     for (int k = 0; k < 2 * HEAD_C; ++k) gw[k] = 0.f;
@@ -560,7 +698,7 @@ inline unsigned grid_for(int64_t n)
 inline unsigned head_blocks(int hw)
 {
     const int b = (hw + 255) / 256;
-    return (unsigned)(b < 1 ? 1 : (b > 64 ? 64 : b));
+    return (unsigned)(b < 1 ? 1 : (b > HEAD_BLOCKS ? HEAD_BLOCKS : b));
 }
 
 }  // namespace
@@ -568,13 +706,13 @@ inline unsigned head_blocks(int hw)
 hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st) { return launch_conv<false>(a, st); }
 hipError_t conv3x3_split(const ConvArgs& a, hipStream_t st) { return launch_conv<true>(a, st); }
 
-hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, const bf16* dz, int cout, int nb,
-                         int h, int w, float* dw, hipStream_t st)
+hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, int mode1, const bf16* dz, int cout,
+                         int nb, int h, int w, float* dw, hipStream_t st)
 {
     const int npix = nb * h * w;
     if (npix == 0) return hipSuccess;
     dim3 grid((cout / 32) * ((c1 + c2) / 32), 9, (npix + WG_PIX - 1) / WG_PIX), blk(256);
-    k_conv3x3_wgrad<<<grid, blk, 0, st>>>(x1, x2, c1, c2, dz, cout, nb, h, w, dw);
+    k_conv3x3_wgrad<<<grid, blk, 0, st>>>(x1, x2, c1, c2, mode1, dz, cout, nb, h, w, dw);
     return hipGetLastError();
 }
 
@@ -622,28 +760,27 @@ hipError_t gelu_bwd(const bf16* da, const bf16* z, int64_t n, bf16* dz, hipStrea
 }
 
 hipError_t head_forward(const bf16* a, int nb, int hw, const float* wh, const float* bh, const float* wm,
-                        float* out, float* sums, hipStream_t st)
+                        float* out, float* part, hipStream_t st)
 {
-    k_fill_f32<<<1, 256, 0, st>>>(sums, 2 * nb, 0.f);
     dim3 grid(head_blocks(hw), nb);
-    k_head_forward<<<grid, 256, 0, st>>>(a, nb, hw, wh, bh, wm, out, sums);
+    k_head_forward<<<grid, 256, 0, st>>>(a, nb, hw, wh, bh, wm, out, part);
     return hipGetLastError();
 }
 
-hipError_t head_project_loss(float* out, const float* sums, const float* target, const float* wm, int nb,
-                             int hw, float* loss, float* dproj, float* sums2, hipStream_t st)
+hipError_t head_project_loss(float* out, const float* part, const float* target, const float* wm, int nb,
+                             int hw, float* loss, float* dproj, float* part2, hipStream_t st)
 {
-    if (target && dproj) k_fill_f32<<<1, 256, 0, st>>>(sums2, 2 * nb, 0.f);
     dim3 grid(head_blocks(hw), nb);
-    k_head_loss<<<grid, 256, 0, st>>>(out, target, wm, nb, hw, sums, loss, dproj, sums2);
+    k_head_loss<<<grid, 256, 0, st>>>(out, target, wm, nb, hw, part, part ? (int)head_blocks(hw) : 0, loss, dproj,
+                                      part2);
     return hipGetLastError();
 }
 
-hipError_t head_backward(const bf16* a, const float* dproj, const float* sums2, const float* wm, int nb,
+hipError_t head_backward(const bf16* a, const float* dproj, const float* part2, const float* wm, int nb,
                          int hw, const float* wh, bf16* da, float* dwh, float* dbh, hipStream_t st)
 {
     dim3 grid(head_blocks(hw), nb);
-    k_head_backward<<<grid, 256, 0, st>>>(a, dproj, sums2, wm, nb, hw, wh, da, dwh, dbh);
+    k_head_backward<<<grid, 256, 0, st>>>(a, dproj, part2, (int)head_blocks(hw), wm, nb, hw, wh, da, dwh, dbh);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ struct ConvArgs {
     const bf16* x1;
     const bf16* x2;
     int c1, c2;
+    int mode1;          // x1 stored at this resolution (0), half (1: upsampled), double (2: pooled)
     int nb, h, w;
     const bf16* wt;
     const float* bias;
@@ -38,9 +39,10 @@ struct ConvArgs {
 hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st);
 hipError_t conv3x3_split(const ConvArgs& a, hipStream_t st);
 
-// dW[tap][co][ci] += sum over pixels of dz[p][co] * x[p shifted by tap][ci]  (FP32, atomics)
-hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, const bf16* dz, int cout, int nb,
-                         int h, int w, float* dw, hipStream_t st);
+// dW[tap][co][ci] += sum over pixels of dz[p][co] * x[p shifted by tap][ci]  (FP32, atomics);
+// x = [x1 | x2] as the forward convolution saw it (mode1 as in ConvArgs)
+hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, int mode1, const bf16* dz, int cout,
+                         int nb, int h, int w, float* dw, hipStream_t st);
 // db[co] += sum over pixels of dz[p][co]
 hipError_t bias_grad(const bf16* dz, int npix, int cout, float* db, hipStream_t st);
 
@@ -57,18 +59,19 @@ hipError_t upsample2_bwd(const bf16* dy, int nb, int h, int w, int c, bf16* dx, 
 // dz = da * GELU'(z), elementwise over n values.
 hipError_t gelu_bwd(const bf16* da, const bf16* z, int64_t n, bf16* dz, hipStream_t st);
 
-// Head: out[p][o] = bh[o] + sum_c a[p][c] wh[o][c] (o < 2, c < 32, FP32 out), then
-// the momentum projection out' = out - sum_i wm_i out_i per image (wm: per-node
-// weights m_i / M, summing to one per image).  sums: [nb][2] scratch, zeroed here.
+// Head: out[p][o] = bh[o] + sum_c a[p][c] wh[o][c] (o < 2, c < 32, FP32 out) and the
+// per-block partial sums part [nb][64][2] of wm_i out_i (wm: per-node weights m_i/M
+// summing to one per image).
 hipError_t head_forward(const bf16* a, int nb, int hw, const float* wh, const float* bh, const float* wm,
-                        float* out, float* sums, hipStream_t st);
-// Projection in place (out -= the image's sums) and, with a target, the loss
+                        float* out, float* part, hipStream_t st);
+// Momentum projection in place, out' = out - sum_i wm_i out_i per image (the sum of
+// the image's partials); with a target, the loss
 // L = (1/nb) sum_images sum_i wm_i |out'_i - target_i|^2 added into *loss and
-// dL/dout' written to dproj with its per-image sums in sums2 [nb][2] (zeroed here).
-hipError_t head_project_loss(float* out, const float* sums, const float* target, const float* wm, int nb,
-                             int hw, float* loss, float* dproj, float* sums2, hipStream_t st);
+// dL/dout' written to dproj with its per-block partial sums in part2 [nb][64][2].
+hipError_t head_project_loss(float* out, const float* part, const float* target, const float* wm, int nb,
+                             int hw, float* loss, float* dproj, float* part2, hipStream_t st);
 // Backward of projection and head: da (BF16, [p][32]), dwh/dbh accumulated (FP32).
-hipError_t head_backward(const bf16* a, const float* dproj, const float* sums2, const float* wm, int nb,
+hipError_t head_backward(const bf16* a, const float* dproj, const float* part2, const float* wm, int nb,
                          int hw, const float* wh, bf16* da, float* dwh, float* dbh, hipStream_t st);
 
 // Adam (FP32): m, v moments; lr_t includes the bias corrections.

@@ -124,6 +124,20 @@ class Surrogate:
         check(self._lib.mpm_surrogate_baseline_loss(self._h, _ptr(s), s.size, _ptr(loss)))
         return float(loss[0])
 
+    def time(self, nb: int, iters: int = 10, train: bool = False) -> float:
+        """Average device ms of a forward (or forward + backward) pass on nb images."""
+        ms = ctypes.c_double(0.0)
+        check(self._lib.mpm_surrogate_time(self._h, int(nb), int(iters), int(train), ctypes.byref(ms)))
+        return float(ms.value)
+
+    def flops(self, nb: int = 1) -> float:
+        """Matrix-core FLOPs of one forward pass on nb images (3x3 convolutions as
+        specified, 32 input channels at the first layer, plus the 1x1 head)."""
+        hw = self.h * self.w
+        spec = [(32, 32, 0), (32, 64, 1), (64, 128, 2), (128, 128, 3), (256, 128, 2), (192, 64, 1), (96, 32, 0)]
+        f = sum(2.0 * 9 * ci * co * hw / 4 ** lvl for ci, co, lvl in spec) + 2.0 * 32 * 2 * hw
+        return f * nb
+
     # -- rollout
     def advance(self, solver, k: int, mode: int, tol: float = 1e-3) -> tuple[int, float]:
         used = ctypes.c_int(0)
