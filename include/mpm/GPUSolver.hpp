@@ -79,4 +79,24 @@ private:
     std::vector<Vec<DIM>> cached_;
 };
 
+// The CPU path's solver (src/cpu/CPUSolver.h:10-27: APIC transfer with an explicit
+// grid force, src/cpu/Grid.cpp and src/cpu/Particles.cpp) on the device: the same
+// interface, with the parameter set's scheme switched to MPM_SCHEME_APIC_FORCE.
+template <int DIM>
+class CPUSolver : public GPUSolver<DIM> {
+public:
+    CPUSolver(const mpm_params& params, int64_t n, const float* x, const float* v, const float* mass,
+              const float* vol, const mpm_options* opts = nullptr)
+        : GPUSolver<DIM>(with_cpu_scheme(params), n, x, v, mass, vol, opts)
+    {
+    }
+
+private:
+    static mpm_params with_cpu_scheme(mpm_params p)
+    {
+        p.scheme = MPM_SCHEME_APIC_FORCE;
+        return p;
+    }
+};
+
 }  // namespace mpm
