@@ -200,7 +200,10 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
             in[r].mode = 0;
-            const int q = tid + 256 * r;
+            // lane l of wave w takes the item's particle 4l + w: the 64 lanes of a wave
+            // spread over the block's cells, so the scatter's LDS adds of one wave
+            // rarely meet on one address (particles are sorted by cell)
+            const int q = 256 * r + ((tid & 63) << 2) + (tid >> 6);
             if (q >= cnt) continue;
             const int p = first + q;
             float x[DIM], v[DIM], C[DIM * DIM], tau[DIM * DIM], Jp;
@@ -387,53 +390,51 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         for (int r = 0; r < RMAX; ++r) {
             if (in[r].mode == 0) continue;
             int base[3] = {0, 0, 0}, rb[3] = {0, 0, 0};
+            float wa[DIM][W], xa[DIM][W];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 base[d] = Spline<ORDER>::base(in[r].xg[d]);
                 rb[d] = base[d] - o[d];
+#pragma unroll
+                for (int s2 = 0; s2 < W; ++s2) {
+                    const float dd = in[r].xg[d] - (float)(base[d] + s2);
+                    wa[d][s2] = Spline<ORDER>::N(dd);
+                    xa[d][s2] = -dd * ph.dx;
+                }
             }
-            // per-lane rotated node order: lanes holding particles of one cell start at
-            // different stencil nodes, so an LDS add rarely meets another lane on its
-            // address; weights are recomputed per node from the operands Stencil::init
-            // uses (bit-identical values, no dynamically indexed arrays)
-            constexpr int NS = DIM == 3 ? W * W * W : W * W;
-            const int r0 = (tid & 63) % NS;
-            int a2 = DIM == 3 ? r0 % W : 0;
-            int a1 = DIM == 3 ? (r0 / W) % W : r0 % W;
-            int a0 = DIM == 3 ? r0 / (W * W) : r0 / W;
             const bool via_tile = in[r].mode == 1;
-#pragma unroll 3
-            for (int sn = 0; sn < NS; ++sn) {
-                int idx[3] = {base[0] + a0, base[1] + a1, DIM == 3 ? base[DIM - 1] + a2 : 0};
-                if (in_grid<DIM>(ph, idx[0], idx[1], idx[2])) {
-                    float xi[DIM], w = 1.f;
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        float dd = in[r].xg[d] - (float)idx[d];
-                        xi[d] = -dd * ph.dx;
-                        w = d == 0 ? Spline<ORDER>::N(dd) : w * Spline<ORDER>::N(dd);
+            for (int a0 = 0; a0 < W; ++a0)
+#pragma unroll
+                for (int a1 = 0; a1 < W; ++a1)
+#pragma unroll
+                    for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                        const int idx[3] = {base[0] + a0, base[1] + a1, DIM == 3 ? base[DIM - 1] + a2 : 0};
+                        if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
+                        float w = wa[0][a0] * wa[1][a1];
+                        float xi[DIM];
+                        xi[0] = xa[0][a0];
+                        xi[1] = xa[1][a1];
+                        if constexpr (DIM == 3) {
+                            w = w * wa[2][a2];
+                            xi[2] = xa[2][a2];
+                        }
+                        float vals[DIM + 1];
+                        node_contrib<DIM>(w, in[r].pm, in[r].mvp, in[r].A, xi, vals);
+                        if (via_tile) {
+                            const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                                   : (rb[1] + a1) * T + (rb[0] + a0);
+                            lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
+#pragma unroll
+                            for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
+                        } else {
+                            float* dstp =
+                                reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
+#pragma unroll
+                            for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
+                            dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
+                        }
                     }
-                    float vals[DIM + 1];
-                    node_contrib<DIM>(w, in[r].pm, in[r].mvp, in[r].A, xi, vals);
-                    if (via_tile) {
-                        int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                         : (rb[1] + a1) * T + (rb[0] + a0);
-                        lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
-#pragma unroll
-                        for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
-                    } else {
-                        float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
-#pragma unroll
-                        for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
-                        dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
-                    }
-                }
-                if constexpr (DIM == 3) {
-                    if (++a2 == W) { a2 = 0; if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; } }
-                } else {
-                    if (++a1 == W) { a1 = 0; if (++a0 == W) a0 = 0; }
-                }
-            }
         }
         __syncthreads();
 
