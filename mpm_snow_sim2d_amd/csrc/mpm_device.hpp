@@ -226,14 +226,16 @@ __device__ inline void svd2(const float* M, float* U, float* sig, float* V)
 __device__ inline void jacobi_pair(float& spp, float& sqq, float& spq, float& sr_p, float& sr_q,
                                    float* V, int p, int q)
 {
-    // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]]
+    // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]]; the
+    // reciprocals and roots use the hardware's single-instruction approximations
+    // (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp)
     float c = 1.0f, s = 0.0f;
     if (!(fabsf(spq) < 1e-10f)) {
-        float tau = (sqq - spp) / (2.0f * spq);
-        float stt = sqrtf(1.0f + tau * tau);
-        float t = 1.0f / (fabsf(tau) + stt);
+        float tau = (sqq - spp) * __builtin_amdgcn_rcpf(2.0f * spq);
+        float stt = __builtin_amdgcn_sqrtf(1.0f + tau * tau);
+        float t = __builtin_amdgcn_rcpf(fabsf(tau) + stt);
         if (tau < 0.0f) t = -t;
-        c = 1.0f / sqrtf(1.0f + t * t);
+        c = __builtin_amdgcn_rsqf(1.0f + t * t);
         s = t * c;
     }
     float Spp = spp, Sqq = sqq, Spq = spq;
@@ -268,10 +270,10 @@ __device__ inline void svd3(const float* M, float* U, float* sig, float* V)
         jacobi_pair(s00, s22, s02, s01, s12, V, 0, 2);   // third index 1: (1,0), (1,2)
         jacobi_pair(s11, s22, s12, s01, s02, V, 1, 2);   // third index 0: (0,1), (0,2)
     }
-    sig[0] = sqrtf(fmaxf(s00, 1e-12f));
-    sig[1] = sqrtf(fmaxf(s11, 1e-12f));
-    sig[2] = sqrtf(fmaxf(s22, 1e-12f));
-    float inv[3] = {1.0f / sig[0], 1.0f / sig[1], 1.0f / sig[2]};
+    sig[0] = __builtin_amdgcn_sqrtf(fmaxf(s00, 1e-12f));
+    sig[1] = __builtin_amdgcn_sqrtf(fmaxf(s11, 1e-12f));
+    sig[2] = __builtin_amdgcn_sqrtf(fmaxf(s22, 1e-12f));
+    float inv[3] = {__builtin_amdgcn_rcpf(sig[0]), __builtin_amdgcn_rcpf(sig[1]), __builtin_amdgcn_rcpf(sig[2])};
     float MV[9];
     matmul<3>(M, V, MV);
 #pragma unroll

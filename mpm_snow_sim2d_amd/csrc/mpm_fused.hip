@@ -27,6 +27,8 @@
 // lane order, and the only rounding is the per-contribution quantisation
 // (<= 2^-31 of the item's largest contribution bound), far below FP32 summation
 // error.
+#include <type_traits>
+
 #include "mpm_kernels.hpp"
 
 namespace mpm {
@@ -232,51 +234,64 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 for (int d = 0; d < DIM; ++d) { nv[d] = 0.f; nx[d] = 0.f; }
 #pragma unroll
                 for (int e = 0; e < DIM * DIM; ++e) nB[e] = 0.f;
+                // whole stencil inside the grid and the tile (nearly every particle):
+                // no per-node bounds tests; the sums are the same in the same order
+                bool inside = tile_ok;
+                {
+                    const int lim[3] = {ph.W, ph.H, ph.D};
 #pragma unroll
-                for (int a0 = 0; a0 < W; ++a0)
+                    for (int a = 0; a < DIM; ++a) inside &= st.base[a] >= 0 && st.base[a] + W - 1 <= lim[a];
+                }
+                auto gather = [&](auto fast) {
 #pragma unroll
-                    for (int a1 = 0; a1 < W; ++a1)
+                    for (int a0 = 0; a0 < W; ++a0)
 #pragma unroll
-                        for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
-                            int idx[3] = {st.base[0] + a0, st.base[1] + a1,
-                                          DIM == 3 ? st.base[DIM - 1] + a2 : 0};
-                            if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
-                            float dp[DIM];
-                            dp[0] = st.d[0][a0] * ph.dx;
-                            dp[1] = st.d[1][a1] * ph.dx;
-                            float w = st.w[0][a0] * st.w[1][a1];
-                            if constexpr (DIM == 3) {
-                                dp[DIM - 1] = st.d[DIM - 1][a2] * ph.dx;
-                                w = w * st.w[DIM - 1][a2];
-                            }
-                            float vF[3], vC[3];
-                            if (tile_ok) {
-                                int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                                 : (rb[1] + a1) * T + (rb[0] + a0);
+                        for (int a1 = 0; a1 < W; ++a1)
 #pragma unroll
-                                for (int d = 0; d < DIM; ++d) {
-                                    vF[d] = s_vel[d][e];
-                                    vC[d] = COULOMB ? s_vel[(DIM + d) % NV][e] : vF[d];
+                            for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                                int idx[3] = {st.base[0] + a0, st.base[1] + a1,
+                                              DIM == 3 ? st.base[DIM - 1] + a2 : 0};
+                                if constexpr (!decltype(fast)::value)
+                                    if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
+                                float dp[DIM];
+                                dp[0] = st.d[0][a0] * ph.dx;
+                                dp[1] = st.d[1][a1] * ph.dx;
+                                float w = st.w[0][a0] * st.w[1][a1];
+                                if constexpr (DIM == 3) {
+                                    dp[DIM - 1] = st.d[DIM - 1][a2] * ph.dx;
+                                    w = w * st.w[DIM - 1][a2];
                                 }
-                            } else {
-                                int addr = node_addr<DIM>(src, idx[0], idx[1], idx[2]);
-                                float4 a4 = src.a[addr];
-                                vF[0] = a4.y; vF[1] = a4.z; vF[2] = a4.w;
-                                if constexpr (COULOMB) {
-                                    float4 c4 = src.vc[addr];
-                                    vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+                                float vF[3], vC[3];
+                                if (decltype(fast)::value || tile_ok) {
+                                    int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                                     : (rb[1] + a1) * T + (rb[0] + a0);
+#pragma unroll
+                                    for (int d = 0; d < DIM; ++d) {
+                                        vF[d] = s_vel[d][e];
+                                        vC[d] = COULOMB ? s_vel[(DIM + d) % NV][e] : vF[d];
+                                    }
                                 } else {
-                                    vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
+                                    int addr = node_addr<DIM>(src, idx[0], idx[1], idx[2]);
+                                    float4 a4 = src.a[addr];
+                                    vF[0] = a4.y; vF[1] = a4.z; vF[2] = a4.w;
+                                    if constexpr (COULOMB) {
+                                        float4 c4 = src.vc[addr];
+                                        vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+                                    } else {
+                                        vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
+                                    }
+                                }
+#pragma unroll
+                                for (int rr = 0; rr < DIM; ++rr) {
+                                    nv[rr] += vF[rr] * w;
+#pragma unroll
+                                    for (int c = 0; c < DIM; ++c) nB[rr * DIM + c] += w * (vF[rr] * (-dp[c]));
+                                    nx[rr] += (ph.dt * vC[rr] + (float)idx[rr] * ph.dx) * w;
                                 }
                             }
-#pragma unroll
-                            for (int rr = 0; rr < DIM; ++rr) {
-                                nv[rr] += vF[rr] * w;
-#pragma unroll
-                                for (int c = 0; c < DIM; ++c) nB[rr * DIM + c] += w * (vF[rr] * (-dp[c]));
-                                nx[rr] += (ph.dt * vC[rr] + (float)idx[rr] * ph.dx) * w;
-                            }
-                        }
+                };
+                if (inside) gather(std::true_type{});
+                else gather(std::false_type{});
                 float M[DIM * DIM], Fo[DIM * DIM], Fh[DIM * DIM], F[DIM * DIM];
 #pragma unroll
                 for (int e = 0; e < DIM * DIM; ++e) {
@@ -403,38 +418,49 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 }
             }
             const bool via_tile = in[r].mode == 1;
+            bool inside = via_tile;
+            {
+                const int lim[3] = {ph.W, ph.H, ph.D};
 #pragma unroll
-            for (int a0 = 0; a0 < W; ++a0)
+                for (int d = 0; d < DIM; ++d) inside &= base[d] >= 0 && base[d] + W - 1 <= lim[d];
+            }
+            auto scatter = [&](auto fast) {
 #pragma unroll
-                for (int a1 = 0; a1 < W; ++a1)
+                for (int a0 = 0; a0 < W; ++a0)
 #pragma unroll
-                    for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
-                        const int idx[3] = {base[0] + a0, base[1] + a1, DIM == 3 ? base[DIM - 1] + a2 : 0};
-                        if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
-                        float w = wa[0][a0] * wa[1][a1];
-                        float xi[DIM];
-                        xi[0] = xa[0][a0];
-                        xi[1] = xa[1][a1];
-                        if constexpr (DIM == 3) {
-                            w = w * wa[2][a2];
-                            xi[2] = xa[2][a2];
+                    for (int a1 = 0; a1 < W; ++a1)
+#pragma unroll
+                        for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                            const int idx[3] = {base[0] + a0, base[1] + a1, DIM == 3 ? base[DIM - 1] + a2 : 0};
+                            if constexpr (!decltype(fast)::value)
+                                if (!in_grid<DIM>(ph, idx[0], idx[1], idx[2])) continue;
+                            float w = wa[0][a0] * wa[1][a1];
+                            float xi[DIM];
+                            xi[0] = xa[0][a0];
+                            xi[1] = xa[1][a1];
+                            if constexpr (DIM == 3) {
+                                w = w * wa[2][a2];
+                                xi[2] = xa[2][a2];
+                            }
+                            float vals[DIM + 1];
+                            node_contrib<DIM>(w, in[r].pm, in[r].mvp, in[r].A, xi, vals);
+                            if (decltype(fast)::value || via_tile) {
+                                const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                                       : (rb[1] + a1) * T + (rb[0] + a0);
+                                lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
+#pragma unroll
+                                for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
+                            } else {
+                                float* dstp =
+                                    reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
+#pragma unroll
+                                for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
+                                dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
+                            }
                         }
-                        float vals[DIM + 1];
-                        node_contrib<DIM>(w, in[r].pm, in[r].mvp, in[r].A, xi, vals);
-                        if (via_tile) {
-                            const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                                   : (rb[1] + a1) * T + (rb[0] + a0);
-                            lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
-#pragma unroll
-                            for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
-                        } else {
-                            float* dstp =
-                                reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
-#pragma unroll
-                            for (int c = 0; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
-                            dst.flags[block_of<DIM>(dst, idx[0], idx[1], idx[2])] = 1u;
-                        }
-                    }
+            };
+            if (inside) scatter(std::true_type{});
+            else scatter(std::false_type{});
         }
         __syncthreads();
 
