@@ -261,6 +261,9 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                     dp[DIM - 1] = st.d[DIM - 1][a2] * ph.dx;
                                     w = w * st.w[DIM - 1][a2];
                                 }
+                                float wdp[DIM];
+#pragma unroll
+                                for (int c = 0; c < DIM; ++c) wdp[c] = w * dp[c];
                                 float vF[3], vC[3];
                                 if (decltype(fast)::value || tile_ok) {
                                     int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 for (int rr = 0; rr < DIM; ++rr) {
                                     nv[rr] += vF[rr] * w;
 #pragma unroll
-                                    for (int c = 0; c < DIM; ++c) nB[rr * DIM + c] += w * (vF[rr] * (-dp[c]));
+                                    for (int c = 0; c < DIM; ++c) nB[rr * DIM + c] -= vF[rr] * wdp[c];
                                     nx[rr] += (ph.dt * vC[rr] + (float)idx[rr] * ph.dx) * w;
                                 }
                             }
@@ -424,6 +427,45 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) inside &= base[d] >= 0 && base[d] + W - 1 <= lim[d];
             }
+            if (inside) {
+                // through the tile, whole stencil inside: the inputs pre-multiplied by
+                // the item's power-of-two scales (exact), one rounding and one LDS add
+                // per component
+                const float pms = in[r].pm * scale_m;
+                float mvs[DIM], As[DIM * DIM];
+#pragma unroll
+                for (int rr = 0; rr < DIM; ++rr) {
+                    mvs[rr] = in[r].mvp[rr] * scale_p;
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) As[rr * DIM + c] = in[r].A[rr * DIM + c] * scale_p;
+                }
+#pragma unroll
+                for (int a0 = 0; a0 < W; ++a0)
+#pragma unroll
+                    for (int a1 = 0; a1 < W; ++a1) {
+                        const float w01 = wa[0][a0] * wa[1][a1];
+#pragma unroll
+                        for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                            float w = w01;
+                            float xi[DIM];
+                            xi[0] = xa[0][a0];
+                            xi[1] = xa[1][a1];
+                            if constexpr (DIM == 3) {
+                                w = w * wa[2][a2];
+                                xi[2] = xa[2][a2];
+                            }
+                            float vals[DIM + 1];
+                            node_contrib<DIM>(w, pms, mvs, As, xi, vals);
+                            const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                                   : (rb[1] + a1) * T + (rb[0] + a0);
+#pragma unroll
+                            for (int c = 0; c <= DIM; ++c)
+                                atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[c][e]),
+                                          (unsigned long long)(long long)__float2int_rn(vals[c]));
+                        }
+                    }
+                continue;
+            }
             auto scatter = [&](auto fast) {
 #pragma unroll
                 for (int a0 = 0; a0 < W; ++a0)
@@ -459,8 +501,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                             }
                         }
             };
-            if (inside) scatter(std::true_type{});
-            else scatter(std::false_type{});
+            scatter(std::false_type{});
         }
         __syncthreads();
 
