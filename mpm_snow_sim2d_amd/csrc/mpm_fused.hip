@@ -20,13 +20,13 @@
 // the result never depends on how stale the binning is.
 //
 // Tile accumulation is exact fixed point.  On gfx950 the LDS float atomic retires
-// about a third of a lane per clock per CU, the 64-bit integer LDS add about eight
+// about a third of a lane per clock per CU, the 32-bit integer LDS add about twelve
 // (measured), so each contribution is scaled by a per-item power of two 2^e with
-// |value| * 2^e <= 2^30 (e from the item's largest per-particle bound), rounded to
-// int32 and summed in int64 LDS words: the tile sum is then exact and independent of
-// lane order, and the only rounding is the per-contribution quantisation
-// (<= 2^-31 of the item's largest contribution bound), far below FP32 summation
-// error.
+// |value| * 2^e <= 2^21 (e from the item's largest per-particle bound), rounded to
+// an integer and summed in int32 LDS words (at most 512 contributions per node, so
+// no overflow): the tile sum is exact and independent of lane order, and the only
+// rounding is the per-contribution quantisation (<= 2^-22 of the item's largest
+// contribution bound) and the one conversion of the sum back to FP32.
 #include <type_traits>
 
 #include "mpm_kernels.hpp"
@@ -58,20 +58,23 @@ __device__ inline bool in_grid(const Phys& ph, int i, int j, int k)
     return ok;
 }
 
-// Fixed-point add of one scaled contribution into an int64 LDS accumulator.
-__device__ inline void lds_add_fixed(long long* p, float v, float scale)
+// Fixed-point add of one contribution into an int32 LDS accumulator: v * scale
+// rounded to the nearest integer (|v| * scale <= 2^21 by the choice of scale).
+__device__ inline void lds_add_fixed(int* p, float v, float scale)
 {
-    int q = __float2int_rn(v * scale);
-    if (q != 0) atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)(long long)q);
+    int q = (int)floorf(fmaf(v, scale, 0.5f));
+    if (q != 0) atomicAdd(p, q);
 }
 
-// Power-of-two scale 2^e with bound * 2^e <= 2^30.
+// Power-of-two scale 2^e with bound * 2^e <= 2^21: an item has at most 512
+// particles, each adding at most once to a node, so a node's sum stays within
+// 2^30 and the int32 accumulator cannot overflow.
 __device__ inline float fixed_scale(float bound)
 {
     if (!(bound > 0.f)) return 1.f;
     int k;
     frexpf(bound, &k);                 // bound < 2^k
-    int e = min(max(30 - k, -126), 126);
+    int e = min(max(21 - k, -126), 126);
     return ldexpf(1.f, e);
 }
 
@@ -114,12 +117,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
     using TS = TileShape<DIM>;
     constexpr int BS = TS::BS, MG = TS::MG, T = TS::T, TN = TS::TN;
     constexpr int W = Spline<ORDER>::W;
-    constexpr int NV = G2P ? (COULOMB ? 2 * DIM : DIM) : 1;
     constexpr int TV = G2P ? TN : 1;
     constexpr int RMAX = 2;                      // particles per thread per item
     constexpr float REACH = ORDER == 3 ? 2.f : 1.5f;   // max |x_i - x_p| / dx per axis
-    __shared__ long long s_acc[DIM + 1][TN];
-    __shared__ float s_vel[NV][TV];
+    __shared__ int s_acc[DIM + 1][TN];
+    // gather velocities, one float4 per node: friction velocity in .xyz (2D: .xy);
+    // with Coulomb walls a second float4 per node holds the collision velocity
+    __shared__ float4 s_vel[COULOMB && G2P ? 2 : 1][TV];
     __shared__ int s_touch[TS::NB];
     __shared__ float s_red[2][4];
 
@@ -182,14 +186,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 #pragma unroll
                 for (int c = 0; c <= DIM; ++c) s_acc[c][e] = 0;
                 if constexpr (G2P) {
-                    s_vel[0][e] = va[k].y;
-                    s_vel[1][e] = va[k].z;
-                    if constexpr (DIM == 3) s_vel[2][e] = va[k].w;
-                    if constexpr (COULOMB) {
-                        s_vel[DIM][e] = vb[k].x;
-                        s_vel[DIM + 1][e] = vb[k].y;
-                        if constexpr (DIM == 3) s_vel[DIM + 2][e] = vb[k].z;
-                    }
+                    s_vel[0][e] = make_float4(va[k].y, va[k].z, va[k].w, 0.f);
+                    if constexpr (COULOMB) s_vel[COULOMB ? 1 : 0][e] = vb[k];
                 }
             }
         }
@@ -268,10 +266,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 if (decltype(fast)::value || tile_ok) {
                                     int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                      : (rb[1] + a1) * T + (rb[0] + a0);
-#pragma unroll
-                                    for (int d = 0; d < DIM; ++d) {
-                                        vF[d] = s_vel[d][e];
-                                        vC[d] = COULOMB ? s_vel[(DIM + d) % NV][e] : vF[d];
+                                    const float4 f4 = s_vel[0][e];
+                                    vF[0] = f4.x; vF[1] = f4.y; vF[2] = f4.z;
+                                    if constexpr (COULOMB) {
+                                        const float4 c4 = s_vel[COULOMB ? 1 : 0][e];
+                                        vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+                                    } else {
+                                        vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
                                     }
                                 } else {
                                     int addr = node_addr<DIM>(src, idx[0], idx[1], idx[2]);
@@ -454,14 +455,20 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 w = w * wa[2][a2];
                                 xi[2] = xa[2][a2];
                             }
-                            float vals[DIM + 1];
-                            node_contrib<DIM>(w, pms, mvs, As, xi, vals);
+                            // round to nearest: floor(w * s + 1/2), the half folded into the FMA
+                            int q[DIM + 1];
+                            q[0] = (int)floorf(fmaf(w, pms, 0.5f));
+#pragma unroll
+                            for (int rr = 0; rr < DIM; ++rr) {
+                                float sm = mvs[rr];
+#pragma unroll
+                                for (int c = 0; c < DIM; ++c) sm += As[rr * DIM + c] * xi[c];
+                                q[1 + rr] = (int)floorf(fmaf(w, sm, 0.5f));
+                            }
                             const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                    : (rb[1] + a1) * T + (rb[0] + a0);
 #pragma unroll
-                            for (int c = 0; c <= DIM; ++c)
-                                atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[c][e]),
-                                          (unsigned long long)(long long)__float2int_rn(vals[c]));
+                            for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc[c][e], q[c]);
                         }
                     }
                 continue;
@@ -508,7 +515,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         // ---- flush the tile: fixed point back to FP32, one global atomic per component
         const float inv_m = 1.f / scale_m, inv_p = 1.f / scale_p;
         for (int e = tid; e < TN; e += 256) {
-            long long acc[DIM + 1];
+            int acc[DIM + 1];
             bool any = false;
 #pragma unroll
             for (int c = 0; c <= DIM; ++c) { acc[c] = s_acc[c][e]; any |= acc[c] != 0; }
@@ -517,9 +524,9 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             tile_coords<DIM>(e, t);
             int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];
             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, i, j, k)]);
-            unsafeAtomicAdd(dstp, (float)((double)acc[0] * (double)inv_m));
+            unsafeAtomicAdd(dstp, (float)acc[0] * inv_m);
 #pragma unroll
-            for (int c = 1; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, (float)((double)acc[c] * (double)inv_p));
+            for (int c = 1; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, (float)acc[c] * inv_p);
             int rbx = (i >> BlockShape<DIM>::SHIFT) - (bc[0] - 1);
             int rby = (j >> BlockShape<DIM>::SHIFT) - (bc[1] - 1);
             int rbz = DIM == 3 ? (k >> BlockShape<DIM>::SHIFT) - (bc[2] - 1) : 0;
