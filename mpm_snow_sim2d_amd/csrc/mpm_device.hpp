@@ -228,16 +228,19 @@ __device__ inline void jacobi_pair(float& spp, float& sqq, float& spq, float& sr
 {
     // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]]; the
     // reciprocals and roots use the hardware's single-instruction approximations
-    // (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp)
-    float c = 1.0f, s = 0.0f;
-    if (!(fabsf(spq) < 1e-10f)) {
-        float tau = (sqq - spp) * __builtin_amdgcn_rcpf(2.0f * spq);
-        float stt = __builtin_amdgcn_sqrtf(1.0f + tau * tau);
-        float t = __builtin_amdgcn_rcpf(fabsf(tau) + stt);
-        if (tau < 0.0f) t = -t;
-        c = __builtin_amdgcn_rsqf(1.0f + t * t);
-        s = t * c;
+    // (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp).  A skipped rotation (c = 1,
+    // s = 0) leaves every value as it is except s_pq = 0, so it is applied as that
+    // alone: a wave whose lanes have all converged passes the pair at once.
+    if (fabsf(spq) < 1e-10f) {
+        spq = 0.0f;
+        return;
     }
+    float tau = (sqq - spp) * __builtin_amdgcn_rcpf(2.0f * spq);
+    float stt = __builtin_amdgcn_sqrtf(1.0f + tau * tau);
+    float t = __builtin_amdgcn_rcpf(fabsf(tau) + stt);
+    if (tau < 0.0f) t = -t;
+    const float c = __builtin_amdgcn_rsqf(1.0f + t * t);
+    const float s = t * c;
     float Spp = spp, Sqq = sqq, Spq = spq;
     spp = c * c * Spp - 2.0f * s * c * Spq + s * s * Sqq;
     sqq = s * s * Spp + 2.0f * s * c * Spq + c * c * Sqq;
