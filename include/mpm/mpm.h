@@ -235,6 +235,37 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
  * buffers: forward only (train = 0) or forward + loss + backward (train = 1). */
 int mpm_surrogate_time(mpm_surrogate* g, int nb, int iters, int train, double* ms);
 
+/* ---------------------------------------------------------------------------
+ * State snapshots (the state GPUSolver::ReadbackParticles and GetParticlePositions
+ * read back, src/gpu/GPUSolver.cpp:204-225, 276-295), single-scene solvers.  File:
+ * "MPMSNAP1", uint32 version 1, uint32 sizeof(mpm_params), mpm_params, int64 n,
+ * int64 substeps, then FP32 x[d][n], v[d][n], C[d*d][n], F[d*d][n], Jp[n], mass[n],
+ * vol[n] in the caller's particle order (little-endian). */
+int mpm_save_snapshot(mpm_solver* s, const char* path);
+/* Header only (no device needed); any pointer may be NULL. */
+int mpm_snapshot_info(const char* path, mpm_params* params, int64_t* n, int64_t* substeps);
+/* A new solver holding the snapshot's parameters and state (opts may be NULL). */
+int mpm_load_snapshot(const char* path, const mpm_options* opts, mpm_solver** out);
+
+/* Headless frames (the reference's Visualize + saveFrame, src/main.cpp:56-160,
+ * 177-205): white points on black, a grey border (2D) or wireframe box (3D) at
+ * BORDER_MARGIN.  2D maps the domain with glOrtho(0, WIDTH, 0, HEIGHT); 3D looks at
+ * the grid centre from a camera orbiting at `distance` (domain units: 300 for the
+ * reference's 200x100x100 grid units) with a perspective projection.  rgb receives
+ * width*height*3 bytes, top row first. */
+typedef struct mpm_camera {
+    float angle_deg;       /* horizontal orbit angle (main.cpp:51)  */
+    float elevation_deg;   /* vertical angle         (main.cpp:52)  */
+    float distance;        /* from the grid centre   (main.cpp:53)  */
+    float fov_deg;         /* vertical field of view (main.cpp:62)  */
+    float z_near, z_far;
+} mpm_camera;
+int mpm_camera_default(mpm_camera* cam);
+int mpm_render_frame(mpm_solver* s, const mpm_camera* cam, int width, int height, int point_size,
+                     unsigned char* rgb);
+/* 8-bit RGB PNG (stored deflate blocks), top row first. */
+int mpm_write_png(const char* path, int width, int height, const unsigned char* rgb);
+
 /* Stable sort of n 32-bit keys on the device (the binning sort on its own, for
  * callers with their own keys).  Host arrays; out_perm[i] = index of the i-th
  * smallest key, ties in input order. key_bits <= 32. */

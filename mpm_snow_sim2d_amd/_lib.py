@@ -36,6 +36,8 @@ EXPORTS = [
     "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_forward", "mpm_surrogate_gradient",
     "mpm_surrogate_record", "mpm_surrogate_get_sample", "mpm_surrogate_train", "mpm_surrogate_baseline_loss",
     "mpm_surrogate_advance", "mpm_surrogate_time",
+    "mpm_save_snapshot", "mpm_snapshot_info", "mpm_load_snapshot",
+    "mpm_camera_default", "mpm_render_frame", "mpm_write_png",
 ]
 
 
@@ -50,6 +52,11 @@ class CParticles(ctypes.Structure):
         ("F", ctypes.c_void_p),
         ("Jp", ctypes.c_void_p),
     ]
+
+
+class CCamera(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("angle_deg", "elevation_deg", "distance", "fov_deg", "z_near", "z_far")]
 
 
 class MPMError(RuntimeError):
@@ -117,6 +124,12 @@ def load() -> ctypes.CDLL:
         "mpm_surrogate_baseline_loss": (i32, [vp, vp, i32, vp]),
         "mpm_surrogate_advance": (i32, [vp, vp, i32, i32, ctypes.c_float, vp, vp]),
         "mpm_surrogate_time": (i32, [vp, i32, i32, i32, P(ctypes.c_double)]),
+        "mpm_save_snapshot": (i32, [vp, ctypes.c_char_p]),
+        "mpm_snapshot_info": (i32, [ctypes.c_char_p, P(Params), P(i64), P(i64)]),
+        "mpm_load_snapshot": (i32, [ctypes.c_char_p, P(COptions), P(vp)]),
+        "mpm_camera_default": (i32, [P(CCamera)]),
+        "mpm_render_frame": (i32, [vp, P(CCamera), i32, i32, i32, vp]),
+        "mpm_write_png": (i32, [ctypes.c_char_p, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libmpm_hip.so")
 SOURCES = ["mpm_kernels.hip", "mpm_fused.hip", "mpm_sort.hip", "mpm_capi.hip", "mpm_unet.hip",
-           "mpm_surrogate_net.hip", "mpm_surrogate.hip"]
+           "mpm_surrogate_net.hip", "mpm_surrogate.hip", "mpm_io.hip"]
 ARCH = os.environ.get("MPM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -54,7 +54,7 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
         cmds.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-    with ThreadPoolExecutor(max_workers=min(7, len(cmds))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

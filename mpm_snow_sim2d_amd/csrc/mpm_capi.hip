@@ -695,7 +695,9 @@ int mpm_step(mpm_solver* s, int n_substeps)
     // four stages
     const bool fused = s->opts.fused && s->params.scheme == MPM_SCHEME_MLS;
     if (!fused && s->nscenes > 1) return fail(MPM_E_STATE, "a batch of scenes runs the fused pipeline only");
-    return fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
+    const int r = fused ? step_fused(s, n_substeps) : step_staged(s, n_substeps);
+    if (r == MPM_OK) s->substeps_done += n_substeps;
+    return r;
 }
 
 int mpm_p2g(mpm_solver* s)

@@ -69,6 +69,7 @@ struct mpm_solver {
     bool items_valid = false;
     bool sorted_keys_in_alt = false;
     int since_sort = 0;
+    int64_t substeps_done = 0;   // substeps advanced since creation (snapshots)
 
     unsigned long long* d_bad = nullptr;
 

@@ -675,6 +675,7 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;
     s->items_valid = false;
+    s->substeps_done += k_substeps;
     if (used) *used = mode;
     return MPM_OK;
 }

@@ -174,6 +174,62 @@ class GPUSolver:
         check(self._lib.mpm_get_kernel_time(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(k)))
         return float(ms.value), int(k.value)
 
+    # -- snapshots and frames
+    def save_snapshot(self, path: str) -> None:
+        """mpm_save_snapshot: parameters, substep count and state (snapshot.py reads it)."""
+        check(self._lib.mpm_save_snapshot(self._h, str(path).encode()))
+
+    @classmethod
+    def from_snapshot(cls, path: str, options: Options | None = None) -> "GPUSolver":
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        opts = COptions.of(options or Options())
+        check(lib.mpm_load_snapshot(str(path).encode(), ctypes.byref(opts), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h, self._lib = h, lib
+        P = Params()
+        n = ctypes.c_int64(0)
+        check(lib.mpm_snapshot_info(str(path).encode(), ctypes.byref(P), ctypes.byref(n), None))
+        self.params, self.n, self.dim = P, int(n.value), P.dim
+        self._keep = []
+        self._cached_positions = np.zeros((self.n, self.dim), np.float32)
+        return self
+
+    def render_frame(self, width: int = 1200, height: int = 600, camera: dict | None = None,
+                     point_size: int = 1) -> np.ndarray:
+        """mpm_render_frame: (height, width, 3) uint8, top row first (Visualize +
+        saveFrame, src/main.cpp:56-160, 177-205)."""
+        from ._lib import CCamera
+        cam = CCamera()
+        check(self._lib.mpm_camera_default(ctypes.byref(cam)))
+        for k, v in (camera or {}).items():
+            setattr(cam, k, float(v))
+        img = np.empty((height, width, 3), np.uint8)
+        check(self._lib.mpm_render_frame(self._h, ctypes.byref(cam), int(width), int(height), int(point_size),
+                                         _ptr(img)))
+        return img
+
+
+def write_png(path: str, rgb: np.ndarray) -> None:
+    """mpm_write_png: an (h, w, 3) uint8 image, top row first."""
+    lib = _lib.load()
+    img = np.ascontiguousarray(rgb, np.uint8)
+    check(lib.mpm_write_png(str(path).encode(), img.shape[1], img.shape[0], _ptr(img)))
+
+
+def save_frames(solver: GPUSolver, out_dir: str, frames: int, substeps_per_frame: int, **render) -> list[str]:
+    """The reference's main loop (src/main.cpp:272-296) without a window: advance,
+    render, save frame_<k>.png.  Returns the written paths."""
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    paths = []
+    for k in range(frames):
+        solver.Step(substeps_per_frame)
+        path = os.path.join(out_dir, f"frame_{k}.png")
+        write_png(path, solver.render_frame(**render))
+        paths.append(path)
+    return paths
+
 
 def _c_particles(ps: ParticleSet, keep: list) -> CParticles:
     d, n = ps.dim, ps.n
@@ -319,4 +375,4 @@ def sort_pairs(keys: np.ndarray, key_bits: int = 32) -> tuple[np.ndarray, np.nda
     return ok, op
 
 
-__all__ = ["BatchSolver", "GPUSolver", "MPMError", "autotune", "sort_pairs"]
+__all__ = ["BatchSolver", "GPUSolver", "MPMError", "autotune", "save_frames", "sort_pairs", "write_png"]
