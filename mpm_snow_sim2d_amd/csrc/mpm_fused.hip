@@ -11,10 +11,11 @@
 // k_tiled<..., G2P=false> is P2G alone from the stored state (first substep).
 //
 // Work decomposition: one work item = the particles binned into one 64-node grid
-// block (at most 512 of them, sorted by cell; two per thread).  The workgroup
-// stages the block's node tile plus a halo (stencil reach + a drift margin of MG
-// cells for particles that moved since the last binning) in LDS: velocities for the
-// gather and a zeroed accumulator for the scatter.  The tile is flushed to HBM with
+// block (at most 512 of them, sorted by cell; at most two per thread, in as few
+// wave passes as the count needs).  The workgroup stages the block's node tile
+// plus a halo (stencil reach + a drift margin of MG cells for particles that moved
+// since the last binning) in LDS: velocities for the gather and a zeroed
+// accumulator for the scatter.  The tile is flushed to HBM with
 // one global float atomic per touched node component.  A particle whose stencil
 // leaves the tile (or whose state is not finite) takes the direct global path, so
 // the result never depends on how stale the binning is.
@@ -197,13 +198,19 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         // ---- phase 1: per particle update (G2P) and what its scatter needs
         P2GIn<DIM> in[RMAX];
         float bound_m = 0.f, bound_p = 0.f;
+        // The item's particles form npass = ceil(cnt / 64) wave passes; pass pi holds
+        // particles pi, pi + npass, pi + 2 npass, ... (one per lane), so only the
+        // passes the item needs run and the 64 lanes of a pass spread over the
+        // block's cells: the scatter's LDS adds of one pass rarely meet on one
+        // address (particles are sorted by cell).  Wave w runs passes
+        // ((w + it) & 3) + 4r; the rotation moves an item's odd pass between waves.
+        const int npass = (cnt + 63) >> 6;
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
             in[r].mode = 0;
-            // lane l of wave w takes the item's particle 4l + w: the 64 lanes of a wave
-            // spread over the block's cells, so the scatter's LDS adds of one wave
-            // rarely meet on one address (particles are sorted by cell)
-            const int q = 256 * r + ((tid & 63) << 2) + (tid >> 6);
+            const int pass = 4 * r + (((tid >> 6) + it) & 3);
+            if (pass >= npass) continue;
+            const int q = pass + npass * (tid & 63);
             if (q >= cnt) continue;
             const int p = first + q;
             float x[DIM], v[DIM], C[DIM * DIM], tau[DIM * DIM], Jp;
