@@ -13,7 +13,9 @@ The other workloads print a line of the same form: `large` (16M particles in fou
 colliding spheres, 512^3 grid, dt 2.5e-5), `batch` (64 independent 2D scenes of
 5k-80k particles on 128^2/256^2 grids in one solver, hipGraph replay, options from
 the autotuner unless --no-autotune) and `snowballs2d` (two 2D snowballs, ~20k
-particles, 128^2 grid, dt 1e-4).
+particles, 128^2 grid, dt 1e-4, hipGraph replay).  Where the timed substeps replay
+hipGraphs, the kernel durations come from a second, eager pass of the same
+substeps from the same state (events are not recorded inside the graphs).
 
 Prints one JSON line.  `roofline` prices the dominant kernel (the fused G2P+P2G
 pass) at 200 algorithmic bytes per particle in 3D, 104 in 2D (FP32 read + write
@@ -254,8 +256,9 @@ def main() -> int:
     elif wl == "snowballs2d":
         P = snow_params(2, 128, 1e-4)
         ps = snowball_disks(seed=0)
+        graph = True   # ~20k particles: launch-bound without graph replay
         desc = (f"plain 2D snow: two snowballs r=0.08 at (0.3,0.5) and (0.7,0.55), +-3 m/s, {ps.n} particles, "
-                "128^2 grid, dx=1/128, dt=1e-4, quadratic MLS-MPM, sticky walls")
+                "128^2 grid, dx=1/128, dt=1e-4, quadratic MLS-MPM, sticky walls, hipGraph replay")
         grid = "128^2"
     else:
         scenes = batch_scenes(64, seed=0)

@@ -230,8 +230,13 @@ __device__ inline void jacobi_pair(float& spp, float& sqq, float& spq, float& sr
     // reciprocals and roots use the hardware's single-instruction approximations
     // (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp).  A skipped rotation (c = 1,
     // s = 0) leaves every value as it is except s_pq = 0, so it is applied as that
-    // alone: a wave whose lanes have all converged passes the pair at once.
-    if (fabsf(spq) < 1e-10f) {
+    // alone: a wave whose lanes have all converged passes the pair at once.  Besides
+    // the reference's absolute test, a pair is taken as converged once
+    // |s_pq| <= 2^-24 sqrt(s_pp s_qq): the rotation would then move the singular
+    // values by less than FP32 resolves, and M = U diag(sigma) V^T with U = M V
+    // diag(1/sigma) holds for any V, so the reconstructed F_E, the rotation U V^T and
+    // the stress change only at round-off level.
+    if (fabsf(spq) < 1e-10f || spq * spq <= 3.5527137e-15f * (spp * sqq)) {
         spq = 0.0f;
         return;
     }

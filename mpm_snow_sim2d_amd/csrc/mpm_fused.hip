@@ -59,11 +59,19 @@ __device__ inline bool in_grid(const Phys& ph, int i, int j, int k)
     return ok;
 }
 
+// a * b rounded to the nearest integer (ties to even), for |a * b| < 2^22: the
+// fused multiply-add with 1.5 * 2^23 rounds once, to a float whose unit in the last
+// place is 1, and the integer sits in the low mantissa bits.
+__device__ inline int round_fixed(float a, float b)
+{
+    return __float_as_int(fmaf(a, b, 12582912.0f)) - 0x4B400000;
+}
+
 // Fixed-point add of one contribution into an int32 LDS accumulator: v * scale
 // rounded to the nearest integer (|v| * scale <= 2^21 by the choice of scale).
 __device__ inline void lds_add_fixed(int* p, float v, float scale)
 {
-    int q = (int)floorf(fmaf(v, scale, 0.5f));
+    const int q = round_fixed(v, scale);
     if (q != 0) atomicAdd(p, q);
 }
 
@@ -77,6 +85,18 @@ __device__ inline float fixed_scale(float bound)
     frexpf(bound, &k);                 // bound < 2^k
     int e = min(max(21 - k, -126), 126);
     return ldexpf(1.f, e);
+}
+
+// Workgroup barrier ordering LDS accesses only: each wave waits for its own LDS
+// operations, not for its global stores and atomics.  The waves of a workgroup
+// exchange data only through LDS here (particle state, grid accumulators and block
+// flags are written for later kernels), so the particle stores of G2P and the
+// global atomics of a tile flush drain while the workgroup moves on.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ inline float wave_max(float v)
@@ -193,7 +213,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             }
         }
         if (tid < TS::NB) s_touch[tid] = 0;
-        __syncthreads();
+        lds_barrier();
 
         // ---- phase 1: per particle update (G2P) and what its scatter needs
         P2GIn<DIM> in[RMAX];
@@ -240,7 +260,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 #pragma unroll
                 for (int e = 0; e < DIM * DIM; ++e) nB[e] = 0.f;
                 // whole stencil inside the grid and the tile (nearly every particle):
-                // no per-node bounds tests; the sums are the same in the same order
+                // no per-node bounds tests, and the stencil sums are contracted one
+                // axis at a time (below)
                 bool inside = tile_ok;
                 {
                     const int lim[3] = {ph.W, ph.H, ph.D};
@@ -301,7 +322,98 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 }
                             }
                 };
-                if (inside) gather(std::true_type{});
+                // Interior particles: the node weight is a product of per-axis
+                // weights, so the sums over the W^DIM nodes factor axis by axis:
+                //   nv   = sum_i w_i v_i
+                //   nB_c = sum_i w_i v_i (x_i - x_p)_c   (one factor w_c -> w_c (x_i - x_p)_c)
+                //   nx   = sum_i w_i (x_i + dt vC_i) = dx sum_i w_i idx_i + dt sum_i w_i vC_i
+                // with sum_i w_i idx_i,c = (sum_s w_c[s] (base_c + s)) prod_{a != c} sum_s w_a[s].
+                // Same sums as the node loop up to the order of the additions
+                // (about 14 FLOPs per node instead of 30).
+                auto gather_separable = [&]() {
+                    float wd[DIM][W];
+#pragma unroll
+                    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) wd[a][s2] = -st.w[a][s2] * (st.d[a][s2] * ph.dx);
+                    float nc[DIM];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) nc[d] = 0.f;
+#pragma unroll
+                    for (int a0 = 0; a0 < W; ++a0) {
+                        // sums over a1 (and a2): S v, D1 = sum wd1 v, D2 = sum wd2 v, Cs vC
+                        float S1[DIM], D1[DIM], D2[DIM], C1[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) { S1[d] = 0.f; D1[d] = 0.f; D2[d] = 0.f; C1[d] = 0.f; }
+#pragma unroll
+                        for (int a1 = 0; a1 < W; ++a1) {
+                            float S2[DIM], E2[DIM], C2[DIM];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) { S2[d] = 0.f; E2[d] = 0.f; C2[d] = 0.f; }
+#pragma unroll
+                            for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
+                                const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
+                                                       : (rb[1] + a1) * T + (rb[0] + a0);
+                                const float4 f4 = s_vel[0][e];
+                                const float vF[3] = {f4.x, f4.y, f4.z};
+                                float vC[3] = {vF[0], vF[1], vF[2]};
+                                if constexpr (COULOMB) {
+                                    const float4 c4 = s_vel[COULOMB ? 1 : 0][e];
+                                    vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+                                }
+                                if constexpr (DIM == 3) {
+                                    const float w2 = st.w[DIM - 1][a2], wd2 = wd[DIM - 1][a2];
+#pragma unroll
+                                    for (int d = 0; d < DIM; ++d) {
+                                        S2[d] += w2 * vF[d];
+                                        E2[d] += wd2 * vF[d];
+                                        if constexpr (COULOMB) C2[d] += w2 * vC[d];
+                                    }
+                                } else {
+#pragma unroll
+                                    for (int d = 0; d < DIM; ++d) { S2[d] = vF[d]; C2[d] = vC[d]; }
+                                }
+                            }
+                            const float w1 = st.w[1][a1], wd1 = wd[1][a1];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) {
+                                S1[d] += w1 * S2[d];
+                                D1[d] += wd1 * S2[d];
+                                if constexpr (DIM == 3) D2[d] += w1 * E2[d];
+                                if constexpr (COULOMB) C1[d] += w1 * C2[d];
+                            }
+                        }
+                        const float w0 = st.w[0][a0], wd0 = wd[0][a0];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            nv[d] += w0 * S1[d];
+                            nB[d * DIM + 0] += wd0 * S1[d];
+                            nB[d * DIM + 1] += w0 * D1[d];
+                            if constexpr (DIM == 3) nB[d * DIM + 2] += w0 * D2[d];
+                            if constexpr (COULOMB) nc[d] += w0 * C1[d];
+                        }
+                    }
+                    float sw[DIM], mi[DIM];
+#pragma unroll
+                    for (int a = 0; a < DIM; ++a) {
+                        sw[a] = 0.f;
+                        mi[a] = 0.f;
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) {
+                            sw[a] += st.w[a][s2];
+                            mi[a] += st.w[a][s2] * (float)(st.base[a] + s2);
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        float m = mi[d];
+#pragma unroll
+                        for (int a = 0; a < DIM; ++a)
+                            if (a != d) m *= sw[a];
+                        nx[d] = m * ph.dx + ph.dt * (COULOMB ? nc[d] : nv[d]);
+                    }
+                };
+                if (inside) gather_separable();
                 else gather(std::false_type{});
                 float M[DIM * DIM], Fo[DIM * DIM], Fh[DIM * DIM], F[DIM * DIM];
 #pragma unroll
@@ -407,7 +519,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         bound_m = wave_max(bound_m);
         bound_p = wave_max(bound_p);
         if ((tid & 63) == 0) { s_red[0][tid >> 6] = bound_m; s_red[1][tid >> 6] = bound_p; }
-        __syncthreads();
+        lds_barrier();
         const float scale_m = fixed_scale(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])));
         const float scale_p = fixed_scale(fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
 
@@ -462,15 +574,15 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 w = w * wa[2][a2];
                                 xi[2] = xa[2][a2];
                             }
-                            // round to nearest: floor(w * s + 1/2), the half folded into the FMA
+                            // round to nearest, the rounding folded into the FMA
                             int q[DIM + 1];
-                            q[0] = (int)floorf(fmaf(w, pms, 0.5f));
+                            q[0] = round_fixed(w, pms);
 #pragma unroll
                             for (int rr = 0; rr < DIM; ++rr) {
                                 float sm = mvs[rr];
 #pragma unroll
                                 for (int c = 0; c < DIM; ++c) sm += As[rr * DIM + c] * xi[c];
-                                q[1 + rr] = (int)floorf(fmaf(w, sm, 0.5f));
+                                q[1 + rr] = round_fixed(w, sm);
                             }
                             const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                    : (rb[1] + a1) * T + (rb[0] + a0);
@@ -517,36 +629,38 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             };
             scatter(std::false_type{});
         }
-        __syncthreads();
+        lds_barrier();
 
-        // ---- flush the tile: fixed point back to FP32, one global atomic per component
+        // ---- flush the tile: fixed point back to FP32, one global atomic per component.
+        // Lanes take (node, component) pairs, component fastest, so one atomic
+        // instruction covers 16 (3D) or 21 (2D) whole nodes: the grid stores a node's
+        // DIM + 1 accumulators contiguously and a block row of nodes contiguously, and
+        // the memory system serves the lanes of one instruction that fall in one cache
+        // line as one request (global atomics are bound by requests, not lanes).
         const float inv_m = 1.f / scale_m, inv_p = 1.f / scale_p;
-        for (int e = tid; e < TN; e += 256) {
-            int acc[DIM + 1];
-            bool any = false;
-#pragma unroll
-            for (int c = 0; c <= DIM; ++c) { acc[c] = s_acc[c][e]; any |= acc[c] != 0; }
-            if (!any) continue;
+        constexpr int NC = DIM + 1;
+        for (int pe = tid; pe < TN * NC; pe += 256) {
+            const int e = pe / NC, c = pe - e * NC;
+            const int acc = s_acc[c][e];
+            if (acc == 0) continue;
             int t[3];
             tile_coords<DIM>(e, t);
             int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];
-            float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, i, j, k)]);
-            unsafeAtomicAdd(dstp, (float)acc[0] * inv_m);
-#pragma unroll
-            for (int c = 1; c <= DIM; ++c) unsafeAtomicAdd(dstp + c, (float)acc[c] * inv_p);
+            float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, i, j, k)]) + c;
+            unsafeAtomicAdd(dstp, (float)acc * (c == 0 ? inv_m : inv_p));
             int rbx = (i >> BlockShape<DIM>::SHIFT) - (bc[0] - 1);
             int rby = (j >> BlockShape<DIM>::SHIFT) - (bc[1] - 1);
             int rbz = DIM == 3 ? (k >> BlockShape<DIM>::SHIFT) - (bc[2] - 1) : 0;
             s_touch[(rbz * 3 + rby) * 3 + rbx] = 1;
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < TS::NB && s_touch[tid]) {
             int rbx = tid % 3, rby = (tid / 3) % 3, rbz = tid / 9;
             int bx = bc[0] - 1 + rbx, by = bc[1] - 1 + rby, bz = DIM == 3 ? bc[2] - 1 + rbz : 0;
             int b = DIM == 3 ? (bz * dst.nby + by) * dst.nbx + bx : by * dst.nbx + bx;
             dst.flags[b] = 1u;
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 

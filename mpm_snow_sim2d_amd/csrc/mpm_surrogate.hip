@@ -59,7 +59,8 @@ int sfail(int code, const std::string& m) { return mpm::detail::set_error(code, 
         if (e_ != hipSuccess) return sfail(MPM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-__device__ inline void block_add(float v, float* dst, float* sh)
+// Sum of v over the workgroup (256 threads), returned to thread 0.
+__device__ inline float block_sum(float v, float* sh)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -67,7 +68,7 @@ __device__ inline void block_add(float v, float* dst, float* sh)
     __syncthreads();
     if (lane == 0) sh[wave] = v;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(dst, sh[0] + sh[1] + sh[2] + sh[3]);
+    return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // Scatter of the features (mode 0) or of the target momentum at the saved
@@ -76,16 +77,38 @@ __device__ inline void block_add(float v, float* dst, float* sh)
 // stencils cover a small box of nodes: the workgroup accumulates that box in LDS
 // and adds it to the image once per node; a workgroup whose box does not fit
 // (after particles have mixed) adds its contributions directly.
-constexpr int CAP_NODES = 1536;   // LDS box capacity in nodes (5 channels, 30 KB)
+//
+// The LDS box accumulates in exact int64 fixed point (the LDS integer adds retire
+// tens of times faster than the LDS float add on gfx950): channel c of the
+// workgroup's particles is scaled by a power of two 2^e_c with
+// max_p |val_c| 2^e_c <= 2^55, and each FP32 contribution w * val * 2^e_c is
+// rounded to an integer, which keeps all 24 bits of any contribution above
+// 2^-31 of the largest.  A node's weights from one particle are <= 9/16 and at most
+// 256 particles add to it, so a sum stays below 2^63; the sum is exact and
+// independent of the order of the additions.
+constexpr int CAP_NODES = 1024;   // LDS box capacity in nodes (5 channels of int64, 40 KB)
+
+__device__ inline float capture_scale(float bound)
+{
+    if (!(bound > 0.f) || !(bound <= 3.0e38f)) return 1.f;   // empty, or not finite: no scaling
+    int k;
+    frexpf(bound, &k);                 // bound < 2^k
+    return ldexpf(1.f, min(max(55 - k, -126), 126));
+}
 
 __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int count, Phys ph, int h, int w,
                                                 float band, float* __restrict__ sample, float* __restrict__ xs,
-                                                SlotInfo* __restrict__ info, int mode)
+                                                float2* __restrict__ part, int mode)
 {
     __shared__ float sh[4];
-    __shared__ float tile[5 * CAP_NODES];
+    __shared__ float s_bound[5][4];
+    __shared__ unsigned long long tile[5 * CAP_NODES];
     __shared__ int box[4];
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    // lane l of wave w takes the workgroup's particle 4l + w: the particles are sorted
+    // by cell, so the 64 lanes of a wave fall in different cells and their LDS adds
+    // rarely meet on one address (same-address adds serialise)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 256 + ((lane << 2) | wave);
     const int n = P.n;
     const size_t hw = (size_t)h * w;
     const int nch = mode == 0 ? 5 : 2;
@@ -125,6 +148,15 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
             val[1] = m * v1;
         }
     }
+    // the workgroup's largest |value| per channel -> fixed-point scales
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        float b = fabsf(val[c]);
+        if (!(b <= 3.0e38f)) b = INFINITY;   // NaN counts as unbounded (fmaxf would drop it)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) b = fmaxf(b, __shfl_xor(b, o));
+        if (lane == 0) s_bound[c][wave] = b;
+    }
     __syncthreads();
     if (live) {
         atomicMin(&box[0], st.base[0]);
@@ -132,14 +164,26 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         atomicMax(&box[2], st.base[0]);
         atomicMax(&box[3], st.base[1]);
     }
+    // a workgroup holding a value that is not finite adds in floating point
+    // (directly), so the non-finite value reaches the image
+    float scale[5], inv_scale[5];
+    bool finite = true;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const float bm = fmaxf(fmaxf(s_bound[c][0], s_bound[c][1]), fmaxf(s_bound[c][2], s_bound[c][3]));
+        finite &= bm <= 3.0e38f;
+        scale[c] = capture_scale(bm);
+        inv_scale[c] = 1.f / scale[c];
+    }
     __syncthreads();
     const bool any_live = box[2] >= box[0];
     const int bx0 = box[0], by0 = box[1];
-    const int bw = any_live ? box[2] - bx0 + 3 : 0, bh = any_live ? box[3] - by0 + 3 : 0;
+    // box extents in 64 bits: bases of non-finite positions saturate at the int range
+    const long long bwl = any_live ? (long long)box[2] - bx0 + 3 : 0, bhl = any_live ? (long long)box[3] - by0 + 3 : 0;
     const size_t ch0 = mode == 0 ? 0 : 5;
-    if (any_live && bw <= CAP_NODES && bh <= CAP_NODES && bw * bh <= CAP_NODES) {
-        const int nn = bw * bh;
-        for (int q = threadIdx.x; q < nn * nch; q += 256) tile[q] = 0.f;
+    if (any_live && finite && bwl <= CAP_NODES && bhl <= CAP_NODES && bwl * bhl <= CAP_NODES) {
+        const int bw = (int)bwl, nn = (int)(bwl * bhl);
+        for (int q = threadIdx.x; q < nn * nch; q += 256) tile[q] = 0ull;
         __syncthreads();
         if (live) {
 #pragma unroll
@@ -148,7 +192,10 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
                 for (int b = 0; b < 3; ++b) {
                     const float wt = st.w[0][a] * st.w[1][b];
                     const int q = (st.base[1] + b - by0) * bw + (st.base[0] + a - bx0);
-                    for (int c = 0; c < nch; ++c) atomicAdd(&tile[c * nn + q], wt * val[c]);
+                    for (int c = 0; c < nch; ++c) {
+                        const long long v = (long long)rintf(wt * (val[c] * scale[c]));
+                        if (v != 0) atomicAdd(&tile[c * nn + q], (unsigned long long)v);
+                    }
                 }
         }
         __syncthreads();
@@ -156,10 +203,11 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
             const int i = bx0 + q % bw, j = by0 + q / bw;
             if (i < 0 || i >= w || j < 0 || j >= h) continue;
             bool any = false;
-            for (int c = 0; c < nch; ++c) any |= tile[c * nn + q] != 0.f;
+            for (int c = 0; c < nch; ++c) any |= tile[c * nn + q] != 0ull;
             if (!any) continue;
             const size_t g = (size_t)j * w + i;
-            for (int c = 0; c < nch; ++c) atomicAdd(sample + (ch0 + c) * hw + g, tile[c * nn + q]);
+            for (int c = 0; c < nch; ++c)
+                atomicAdd(sample + (ch0 + c) * hw + g, (float)(long long)tile[c * nn + q] * inv_scale[c]);
         }
     } else if (live) {
 #pragma unroll
@@ -173,15 +221,18 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
                 for (int c = 0; c < nch; ++c) atomicAdd(sample + (ch0 + c) * hw + g, wt * val[c]);
             }
     }
+    // per-workgroup sums of particle mass and wall-band mass (reduced by
+    // k_info_reduce: no same-address atomics across workgroups)
     if (mode == 0) {
-        block_add(pm, &info->particle_mass, sh);
-        block_add(inband, &info->band_mass, sh);
+        const float a = block_sum(pm, sh);
+        const float b = block_sum(inband, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = make_float2(a, b);
     }
 }
 
-// Momentum sums to per-node means (channels 1-4 in mode 0, 5-6 in mode 1), and the
-// image's grid mass.
-__global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ sample, int hw, SlotInfo* info,
+// Momentum sums to per-node means (channels 1-4 in mode 0, 5-6 in mode 1), and
+// per-workgroup sums of the image's grid mass.
+__global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ sample, int hw, float* __restrict__ part,
                                                        int mode)
 {
     __shared__ float sh[4];
@@ -196,7 +247,32 @@ __global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ samp
             for (int c = 5; c <= 6; ++c) sample[(size_t)c * hw + q] *= inv;
         }
     }
-    if (mode == 0) block_add(m, &info->grid_mass, sh);
+    if (mode == 0) {
+        const float a = block_sum(m, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = a;
+    }
+}
+
+// The slot's particle, wall-band and grid mass from the workgroup sums.
+__global__ void __launch_bounds__(256) k_info_reduce(const float2* __restrict__ pp, int npp,
+                                                    const float* __restrict__ pg, int npg, SlotInfo* info)
+{
+    __shared__ float sh[4];
+    float a = 0.f, b = 0.f, c = 0.f;
+    for (int i = threadIdx.x; i < npp; i += 256) {
+        const float2 v = pp[i];
+        a += v.x;
+        b += v.y;
+    }
+    for (int i = threadIdx.x; i < npg; i += 256) c += pg[i];
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    c = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        info->particle_mass = a;
+        info->band_mass = b;
+        info->grid_mass = c;
+    }
 }
 
 // Batch of network inputs, projection weights and targets from dataset slots.
@@ -312,23 +388,19 @@ __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, in
     P.Jp[p] = P.Jp[p] * ratio;
 }
 
-__global__ void k_zero_info(SlotInfo* info)
+__global__ void k_init_info(SlotInfo* info, float T, float g, float m_ref)
 {
     info->particle_mass = 0.f;
     info->grid_mass = 0.f;
     info->band_mass = 0.f;
+    info->T = T;
+    info->gravity = g;
+    info->m_ref = m_ref;
 }
 
 __global__ void __launch_bounds__(256) k_zero(float* p, int64_t n)
 {
     for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.f;
-}
-
-__global__ void k_set_info(SlotInfo* info, float T, float g, float m_ref)
-{
-    info->T = T;
-    info->gravity = g;
-    info->m_ref = m_ref;
 }
 
 __global__ void k_copy_info(SlotInfo* dst, const SlotInfo* src) { *dst = *src; }
@@ -344,6 +416,8 @@ struct mpm_surrogate {
     SlotInfo* info = nullptr;      // capacity + 1
     float* xs = nullptr;           // saved positions by original particle index
     int64_t xs_cap = 0;
+    float2* part_p = nullptr;      // per-workgroup particle / wall-band mass of a capture (xs_cap / 256 + 1)
+    float* part_g = nullptr;       // per-workgroup grid mass of a capture (h * w / 256 + 1)
     bf16* in = nullptr;
     float* wm = nullptr;
     float* target = nullptr;
@@ -370,9 +444,12 @@ int ensure_xs(mpm_surrogate* g, mpm_solver* s)
 {
     if (g->xs_cap >= s->n) return MPM_OK;
     if (g->xs) S_TRY(hipFree(g->xs));
+    if (g->part_p) S_TRY(hipFree(g->part_p));
     g->xs = nullptr;
+    g->part_p = nullptr;
     g->xs_cap = 0;
     S_TRY(hipMalloc(&g->xs, (size_t)std::max(s->n, 1) * 2 * sizeof(float)));
+    S_TRY(hipMalloc(&g->part_p, ((size_t)s->n / 256 + 1) * sizeof(float2)));
     g->xs_cap = s->n;
     return MPM_OK;
 }
@@ -393,14 +470,15 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     const int first = (int)s->p_off[scene], count = (int)(s->p_off[scene + 1] - s->p_off[scene]);
     if (mode == 0) {
         k_zero<<<std::min((NCH * hw + 255) / 256, 4096), 256, 0, g->st>>>(sample, (int64_t)NCH * hw);
-        k_zero_info<<<1, 1, 0, g->st>>>(info);
-        k_set_info<<<1, 1, 0, g->st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
+        k_init_info<<<1, 1, 0, g->st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
     }
     const float band = P.BORDER_MARGIN + 2.0f;
+    const int nbp = (count + 255) / 256, nbg = (hw + 255) / 256;
     if (count > 0)
-        k_capture<<<(count + 255) / 256, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band,
-                                                         sample, g->xs, info, mode);
-    k_capture_finish<<<(hw + 255) / 256, 256, 0, g->st>>>(sample, hw, info, mode);
+        k_capture<<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample, g->xs,
+                                          g->part_p, mode);
+    k_capture_finish<<<nbg, 256, 0, g->st>>>(sample, hw, g->part_g, mode);
+    if (mode == 0) k_info_reduce<<<1, 256, 0, g->st>>>(g->part_p, nbp, g->part_g, nbg, info);
     S_TRY(hipGetLastError());
     return MPM_OK;
 }
@@ -446,7 +524,8 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
         hipMalloc(&g->wm, (size_t)max_batch * hw * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->target, (size_t)max_batch * hw * 2 * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->slots_d, (size_t)max_batch * sizeof(int)) != hipSuccess ||
-        hipMalloc(&g->loss_d, 256) != hipSuccess)
+        hipMalloc(&g->loss_d, 256) != hipSuccess ||
+        hipMalloc(&g->part_g, (hw / 256 + 1) * sizeof(float)) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "surrogate allocation failed"));
     if (hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
         hipMemsetAsync(g->in, 0, (size_t)max_batch * hw * 32 * sizeof(bf16), g->st) != hipSuccess ||
@@ -466,6 +545,8 @@ int mpm_surrogate_destroy(mpm_surrogate* g)
     (void)hipFree(g->data);
     (void)hipFree(g->info);
     (void)hipFree(g->xs);
+    (void)hipFree(g->part_p);
+    (void)hipFree(g->part_g);
     (void)hipFree(g->in);
     (void)hipFree(g->wm);
     (void)hipFree(g->target);
