@@ -218,11 +218,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         // ---- phase 1: per particle update (G2P) and what its scatter needs
         P2GIn<DIM> in[RMAX];
         float bound_m = 0.f, bound_p = 0.f;
-        // The item's particles form npass = ceil(cnt / 64) wave passes; pass pi holds
-        // particles pi, pi + npass, pi + 2 npass, ... (one per lane), so only the
-        // passes the item needs run and the 64 lanes of a pass spread over the
-        // block's cells: the scatter's LDS adds of one pass rarely meet on one
-        // address (particles are sorted by cell).  Wave w runs passes
+        // The item's particles form npass = ceil(cnt / 64) wave passes, so only the
+        // passes the item needs run.  3D: a pass holds 64 consecutive particles
+        // (coalesced loads and stores of the particle state, which measured faster
+        // than spreading lanes over cells).  2D: pass pi holds particles pi,
+        // pi + npass, pi + 2 npass, ..., so the 64 lanes spread over the block's cells
+        // and the scatter's LDS adds of one pass rarely meet on one address (particles
+        // are sorted by cell), which measured faster there.  Wave w runs passes
         // ((w + it) & 3) + 4r; the rotation moves an item's odd pass between waves.
         const int npass = (cnt + 63) >> 6;
 #pragma unroll
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             in[r].mode = 0;
             const int pass = 4 * r + (((tid >> 6) + it) & 3);
             if (pass >= npass) continue;
-            const int q = pass + npass * (tid & 63);
+            const int q = DIM == 3 ? pass * 64 + (tid & 63) : pass + npass * (tid & 63);
             if (q >= cnt) continue;
             const int p = first + q;
             float x[DIM], v[DIM], C[DIM * DIM], tau[DIM * DIM], Jp;
