@@ -78,14 +78,17 @@ __device__ inline float block_sum(float v, float* sh)
 // and adds it to the image once per node; a workgroup whose box does not fit
 // (after particles have mixed) adds its contributions directly.
 //
-// The LDS box accumulates in exact int64 fixed point (the LDS integer adds retire
-// tens of times faster than the LDS float add on gfx950): channel c of the
-// workgroup's particles is scaled by a power of two 2^e_c with
-// max_p |val_c| 2^e_c <= 2^55, and each FP32 contribution w * val * 2^e_c is
-// rounded to an integer, which keeps all 24 bits of any contribution above
-// 2^-31 of the largest.  A node's weights from one particle are <= 9/16 and at most
-// 256 particles add to it, so a sum stays below 2^63; the sum is exact and
-// independent of the order of the additions.
+// The LDS box accumulates in exact fixed point (the LDS integer add retires tens of
+// times faster than the LDS float add on gfx950): channel c of the workgroup's
+// particles is scaled by a power of two 2^e_c with max_p |val_c| 2^e_c <= 2^45, and
+// each FP32 contribution x = rint(w * val * 2^e_c) becomes an int64 through the exact
+// split hi = floor(x / 2^22), lo = x - hi 2^22 (two single-instruction FP32->int32
+// conversions; there is no FP32->int64 instruction) and is added to an int64 word.
+// A node's weights from one particle are <= 9/16 and at most 256 particles add to
+// it, so a sum stays below 2^53: exact in FP64 when converted back, and independent
+// of the order of the additions; a
+// contribution loses nothing above 2^-22 of the workgroup's largest, and at most
+// half a unit of 2^-45 of it below.
 constexpr int CAP_NODES = 1024;   // LDS box capacity in nodes (5 channels of int64, 40 KB)
 
 __device__ inline float capture_scale(float bound)
@@ -93,17 +96,40 @@ __device__ inline float capture_scale(float bound)
     if (!(bound > 0.f) || !(bound <= 3.0e38f)) return 1.f;   // empty, or not finite: no scaling
     int k;
     frexpf(bound, &k);                 // bound < 2^k
-    return ldexpf(1.f, min(max(55 - k, -126), 126));
+    return ldexpf(1.f, min(max(45 - k, -126), 126));
 }
 
+__device__ inline float wave_max_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ inline int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ inline int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int count, Phys ph, int h, int w,
                                                 float band, float* __restrict__ sample, float* __restrict__ xs,
-                                                float2* __restrict__ part, int mode)
+                                                float2* __restrict__ part)
 {
+    constexpr int NCAP = MODE == 0 ? 5 : 2;   // channels scattered
     __shared__ float sh[4];
-    __shared__ float s_bound[5][4];
-    __shared__ unsigned long long tile[5 * CAP_NODES];
-    __shared__ int box[4];
+    __shared__ float s_bound[NCAP][4];
+    __shared__ int s_box[4][4];
+    __shared__ unsigned long long tile[NCAP * CAP_NODES];
     // lane l of wave w takes the workgroup's particle 4l + w: the particles are sorted
     // by cell, so the 64 lanes of a wave fall in different cells and their LDS adds
     // rarely meet on one address (same-address adds serialise)
@@ -111,19 +137,17 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
     const int k = blockIdx.x * 256 + ((lane << 2) | wave);
     const int n = P.n;
     const size_t hw = (size_t)h * w;
-    const int nch = mode == 0 ? 5 : 2;
     float pm = 0.f, inband = 0.f;
-    float val[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float val[NCAP];
+#pragma unroll
+    for (int c = 0; c < NCAP; ++c) val[c] = 0.f;
     Stencil<2, 2> st;
     const bool live = k < count;
-    if (threadIdx.x == 0) {
-        box[0] = INT_MAX; box[1] = INT_MAX; box[2] = INT_MIN; box[3] = INT_MIN;
-    }
     if (live) {
         const int p = first + k;
         const uint32_t id = P.pid[p];
         float x0 = P.x[p], x1 = P.x[n + p];
-        if (mode == 0) {
+        if constexpr (MODE == 0) {
             xs[2 * (size_t)id] = x0;
             xs[2 * (size_t)id + 1] = x1;
         } else {
@@ -136,7 +160,7 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         if (xg[0] < band || xg[0] > (float)ph.W - band || xg[1] < band || xg[1] > (float)ph.H - band) inband = m;
         st.init(xg);
         const float v0 = P.v[p], v1 = P.v[n + p];
-        if (mode == 0) {
+        if constexpr (MODE == 0) {
             const float F[4] = {P.F[p], P.F[n + p], P.F[2 * n + p], P.F[3 * n + p]};
             val[0] = m;
             val[1] = m * v0;
@@ -148,52 +172,83 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
             val[1] = m * v1;
         }
     }
-    // the workgroup's largest |value| per channel -> fixed-point scales
+    // the workgroup's largest |value| per channel (-> fixed-point scales) and the box
+    // of its stencils, reduced per wave, then across the four waves
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NCAP; ++c) {
         float b = fabsf(val[c]);
         if (!(b <= 3.0e38f)) b = INFINITY;   // NaN counts as unbounded (fmaxf would drop it)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) b = fmaxf(b, __shfl_xor(b, o));
+        b = wave_max_f(b);
         if (lane == 0) s_bound[c][wave] = b;
     }
-    __syncthreads();
-    if (live) {
-        atomicMin(&box[0], st.base[0]);
-        atomicMin(&box[1], st.base[1]);
-        atomicMax(&box[2], st.base[0]);
-        atomicMax(&box[3], st.base[1]);
+    {
+        const int b0 = wave_min_i(live ? st.base[0] : INT_MAX), b1 = wave_min_i(live ? st.base[1] : INT_MAX);
+        const int b2 = wave_max_i(live ? st.base[0] : INT_MIN), b3 = wave_max_i(live ? st.base[1] : INT_MIN);
+        if (lane == 0) {
+            s_box[0][wave] = b0;
+            s_box[1][wave] = b1;
+            s_box[2][wave] = b2;
+            s_box[3][wave] = b3;
+        }
     }
+    __syncthreads();
     // a workgroup holding a value that is not finite adds in floating point
     // (directly), so the non-finite value reaches the image
-    float scale[5], inv_scale[5];
+    float scale[NCAP];
+    double inv_scale[NCAP];
     bool finite = true;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < NCAP; ++c) {
         const float bm = fmaxf(fmaxf(s_bound[c][0], s_bound[c][1]), fmaxf(s_bound[c][2], s_bound[c][3]));
         finite &= bm <= 3.0e38f;
         scale[c] = capture_scale(bm);
-        inv_scale[c] = 1.f / scale[c];
+        inv_scale[c] = 1.0 / (double)scale[c];
     }
-    __syncthreads();
-    const bool any_live = box[2] >= box[0];
-    const int bx0 = box[0], by0 = box[1];
+    const int bx0 = min(min(s_box[0][0], s_box[0][1]), min(s_box[0][2], s_box[0][3]));
+    const int by0 = min(min(s_box[1][0], s_box[1][1]), min(s_box[1][2], s_box[1][3]));
+    const int bx1 = max(max(s_box[2][0], s_box[2][1]), max(s_box[2][2], s_box[2][3]));
+    const int by1 = max(max(s_box[3][0], s_box[3][1]), max(s_box[3][2], s_box[3][3]));
+    const bool any_live = bx1 >= bx0;
     // box extents in 64 bits: bases of non-finite positions saturate at the int range
-    const long long bwl = any_live ? (long long)box[2] - bx0 + 3 : 0, bhl = any_live ? (long long)box[3] - by0 + 3 : 0;
-    const size_t ch0 = mode == 0 ? 0 : 5;
+    const long long bwl = any_live ? (long long)bx1 - bx0 + 3 : 0, bhl = any_live ? (long long)by1 - by0 + 3 : 0;
+    const size_t ch0 = MODE == 0 ? 0 : 5;
     if (any_live && finite && bwl <= CAP_NODES && bhl <= CAP_NODES && bwl * bhl <= CAP_NODES) {
         const int bw = (int)bwl, nn = (int)(bwl * bhl);
-        for (int q = threadIdx.x; q < nn * nch; q += 256) tile[q] = 0ull;
+        for (int q = threadIdx.x; q < nn * NCAP; q += 256) tile[q] = 0ull;
         __syncthreads();
         if (live) {
+            float vs[NCAP];
+#pragma unroll
+            for (int c = 0; c < NCAP; ++c) vs[c] = val[c] * scale[c];
+            // each lane walks its 3x3 stencil from a start rotated by its lane index,
+            // so lanes whose particles share a cell (dense or compressed regions) add
+            // to different nodes at each step
+            float wx[3], wy[3];
+            int ox[3], oy[3];
+            {
+                const int r0 = lane % 3, r1 = (lane / 3) % 3;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int ax = a + r0 >= 3 ? a + r0 - 3 : a + r0;
+                    const int ay = a + r1 >= 3 ? a + r1 - 3 : a + r1;
+                    wx[a] = ax == 0 ? st.w[0][0] : ax == 1 ? st.w[0][1] : st.w[0][2];
+                    wy[a] = ay == 0 ? st.w[1][0] : ay == 1 ? st.w[1][1] : st.w[1][2];
+                    ox[a] = st.base[0] + ax - bx0;
+                    oy[a] = (st.base[1] + ay - by0) * bw;
+                }
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const float wt = st.w[0][a] * st.w[1][b];
-                    const int q = (st.base[1] + b - by0) * bw + (st.base[0] + a - bx0);
-                    for (int c = 0; c < nch; ++c) {
-                        const long long v = (long long)rintf(wt * (val[c] * scale[c]));
+                    const float wt = wx[a] * wy[b];
+                    const int q = oy[b] + ox[a];
+#pragma unroll
+                    for (int c = 0; c < NCAP; ++c) {
+                        const float x = rintf(wt * vs[c]);
+                        const float hi = floorf(x * 2.384185791015625e-7f);   // x / 2^22
+                        const float lo = fmaf(-hi, 4194304.0f, x);            // exact, in [0, 2^22)
+                        const long long v = (long long)(int)hi * 4194304ll + (long long)(int)lo;
                         if (v != 0) atomicAdd(&tile[c * nn + q], (unsigned long long)v);
                     }
                 }
@@ -202,12 +257,14 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         for (int q = threadIdx.x; q < nn; q += 256) {
             const int i = bx0 + q % bw, j = by0 + q / bw;
             if (i < 0 || i >= w || j < 0 || j >= h) continue;
-            bool any = false;
-            for (int c = 0; c < nch; ++c) any |= tile[c * nn + q] != 0ull;
-            if (!any) continue;
             const size_t g = (size_t)j * w + i;
-            for (int c = 0; c < nch; ++c)
-                atomicAdd(sample + (ch0 + c) * hw + g, (float)(long long)tile[c * nn + q] * inv_scale[c]);
+#pragma unroll
+            for (int c = 0; c < NCAP; ++c) {
+                const long long t = (long long)tile[c * nn + q];
+                if (t == 0) continue;
+                const double v = (double)t * inv_scale[c];
+                atomicAdd(sample + (ch0 + c) * hw + g, (float)v);
+            }
         }
     } else if (live) {
 #pragma unroll
@@ -218,12 +275,13 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
                 if (i < 0 || i >= w || j < 0 || j >= h) continue;
                 const float wt = st.w[0][a] * st.w[1][b];
                 const size_t g = (size_t)j * w + i;
-                for (int c = 0; c < nch; ++c) atomicAdd(sample + (ch0 + c) * hw + g, wt * val[c]);
+#pragma unroll
+                for (int c = 0; c < NCAP; ++c) atomicAdd(sample + (ch0 + c) * hw + g, wt * val[c]);
             }
     }
     // per-workgroup sums of particle mass and wall-band mass (reduced by
     // k_info_reduce: no same-address atomics across workgroups)
-    if (mode == 0) {
+    if constexpr (MODE == 0) {
         const float a = block_sum(pm, sh);
         const float b = block_sum(inband, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = make_float2(a, b);
@@ -474,9 +532,14 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     }
     const float band = P.BORDER_MARGIN + 2.0f;
     const int nbp = (count + 255) / 256, nbg = (hw + 255) / 256;
-    if (count > 0)
-        k_capture<<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample, g->xs,
-                                          g->part_p, mode);
+    if (count > 0) {
+        if (mode == 0)
+            k_capture<0><<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
+                                                 g->xs, g->part_p);
+        else
+            k_capture<1><<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
+                                                 g->xs, g->part_p);
+    }
     k_capture_finish<<<nbg, 256, 0, g->st>>>(sample, hw, g->part_g, mode);
     if (mode == 0) k_info_reduce<<<1, 256, 0, g->st>>>(g->part_p, nbp, g->part_g, nbg, info);
     S_TRY(hipGetLastError());
