@@ -275,6 +275,143 @@ hipError_t launch_conv(const ConvArgs& a, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
+// LDS-tiled forward 3x3 convolution.  A workgroup computes TH x 16 pixels (rows x
+// columns) of one image for 16*NT output channels.  Per 32-channel input chunk it
+// stages the (TH+2) x 18 pixel patch (tile and one-pixel halo, zero outside the
+// image) and the 9 x 16*NT x 32 weights in LDS once, then each wave runs all nine
+// taps over its TH/4 rows from LDS: every staged value is used by up to nine taps
+// (input) or all TH/4 x 16 pixels of the wave (weights) instead of being fetched
+// again per tap.  Operands: weights as A (rows = output channels), pixels as B, as
+// in k_conv3x3; a pixel's or weight row's 32 channels sit at a pitch of 40 BF16
+// (80 bytes) so that the 16-byte fragment reads of 16 neighbouring rows spread
+// over the LDS banks.
+template <int TH, int NT>
+__global__ void __launch_bounds__(256) k_conv3x3_lds(ConvArgs a, int tiles_x, int tiles_y, int co_blocks)
+{
+    constexpr int TW = 16, PW = TW + 2, PH = TH + 2;
+    constexpr int MT = TH / 4;           // rows per wave
+    constexpr int CP = 40;               // LDS pitch (bf16) of a pixel's / weight row's 32 channels
+    constexpr int COW = 16 * NT;         // output channels per workgroup
+    __shared__ __attribute__((aligned(16))) bf16 s_in[PH * PW * CP];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[9 * COW * CP];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    int bid = blockIdx.x;
+    const int cblk = bid % co_blocks;
+    bid /= co_blocks;
+    const int tx = bid % tiles_x;
+    bid /= tiles_x;
+    const int ty = bid % tiles_y;
+    const int b = bid / tiles_y;
+    const int y0 = ty * TH, x0 = tx * TW, co0 = cblk * COW;
+    const int cin = a.c1 + a.c2;
+    const int nchunk = cin >> 5;
+
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // chunk staging through registers: the global loads of chunk ch + 1 are in
+    // flight while the waves run chunk ch from LDS
+    constexpr int NV_IN = (PH * PW * 4 + 255) / 256, NV_W = (9 * COW * 4 + 255) / 256;
+    bf16x8 rin[NV_IN], rw[NV_W];
+    auto fetch = [&](int ch) {
+        const int ci0 = ch << 5;
+        const bool first = ci0 < a.c1;
+        const bf16* xs = first ? a.x1 : a.x2;
+        const int cs = first ? a.c1 : a.c2;
+        const int cc = first ? ci0 : ci0 - a.c1;
+        const int md = first ? a.mode1 : 0;
+#pragma unroll
+        for (int i = 0; i < NV_IN; ++i) {
+            const int v = threadIdx.x + 256 * i;
+            const int pix = v >> 2, q = v & 3;
+            const int py = pix / PW, px = pix - py * PW;
+            const int yy = y0 + py - 1, xx = x0 + px - 1;
+            rin[i] = zero8();
+            if (v < PH * PW * 4 && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+                rin[i] = load_in(xs, cs, cc + 8 * q, b, yy, xx, a.h, a.w, md);
+        }
+#pragma unroll
+        for (int i = 0; i < NV_W; ++i) {
+            const int v = threadIdx.x + 256 * i;
+            const int row = v >> 2, q = v & 3;   // row = tap * COW + channel
+            const int tap = row / COW, co = row - tap * COW;
+            rw[i] = zero8();
+            if (v < 9 * COW * 4)
+                rw[i] = *reinterpret_cast<const bf16x8*>(a.wt + ((size_t)tap * a.cout + co0 + co) * cin + ci0 + 8 * q);
+        }
+    };
+    fetch(0);
+    for (int ch = 0; ch < nchunk; ++ch) {
+#pragma unroll
+        for (int i = 0; i < NV_IN; ++i) {
+            const int v = threadIdx.x + 256 * i;
+            if (v < PH * PW * 4) *reinterpret_cast<bf16x8*>(&s_in[(v >> 2) * CP + 8 * (v & 3)]) = rin[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NV_W; ++i) {
+            const int v = threadIdx.x + 256 * i;
+            if (v < 9 * COW * 4) *reinterpret_cast<bf16x8*>(&s_w[(v >> 2) * CP + 8 * (v & 3)]) = rw[i];
+        }
+        __syncthreads();
+        if (ch + 1 < nchunk) fetch(ch + 1);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;   // patch offsets of the tap (0..2)
+            bf16x8 fw[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                fw[t] = *reinterpret_cast<const bf16x8*>(&s_w[(tap * COW + 16 * t + r) * CP + 8 * g]);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int prow = wave * MT + mt + dy;
+                const bf16x8 fx = *reinterpret_cast<const bf16x8*>(&s_in[(prow * PW + r + dx) * CP + 8 * g]);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[t], fx, acc[mt][t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // D[4g+e][r] of (mt, t): output channel co0 + 16t + 4g + e of pixel
+    // (y0 + wave*MT + mt, x0 + r); four channels go out in one 8-byte store
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + wave * MT + mt, x = x0 + r;
+        if (y >= a.h || x >= a.w) continue;
+        const size_t p = ((size_t)b * a.h + y) * a.w + x;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int cb = co0 + 16 * t + 4 * g;
+            bf16x4 zo, ao;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = acc[mt][t][e] + (a.bias ? a.bias[cb + e] : 0.f);
+                zo[e] = (bf16)v;
+                ao[e] = (bf16)gelu(v);
+            }
+            if (a.z) *reinterpret_cast<bf16x4*>(a.z + p * a.cout + cb) = zo;
+            if (a.a) *reinterpret_cast<bf16x4*>(a.a + p * a.cout + cb) = ao;
+        }
+    }
+}
+
+template <int TH>
+hipError_t launch_conv_lds(const ConvArgs& a, hipStream_t st)
+{
+    const int tiles_x = (a.w + 15) / 16, tiles_y = (a.h + TH - 1) / TH, co_blocks = a.cout / 32;
+    const long long nwg = (long long)a.nb * tiles_y * tiles_x * co_blocks;
+    k_conv3x3_lds<TH, 2><<<(unsigned)nwg, 256, 0, st>>>(a, tiles_x, tiles_y, co_blocks);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: per tap a GEMM dW[co][ci] = sum_p dz[p][co] x[p+tap][ci].
 // Workgroup: one tap, a 32x32 (co, ci) tile, a range of pixels; each wave takes
 // 32-pixel slabs, transposes them through its own LDS region ([channel][pixel])
@@ -703,7 +840,22 @@ inline unsigned head_blocks(int hw)
 
 }  // namespace
 
-hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st) { return launch_conv<false>(a, st); }
+// Forward convolutions: the LDS-tiled kernel with the tallest tile (most reuse of
+// the staged weights) that still gives >= 1024 workgroups (four per CU, so that
+// their staging and matrix work overlap), 4-row tiles for smaller layers down to
+// 64 workgroups (measured faster than the register-streaming kernel there), and
+// the register-streaming kernel with its split reduction below that.
+hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st)
+{
+    if (a.nb * a.h * a.w == 0) return hipSuccess;
+    auto wgs = [&](int th) { return (long long)a.nb * ((a.h + th - 1) / th) * ((a.w + 15) / 16) * (a.cout / 32); };
+    if (a.cout % 32 == 0 && (a.c1 + a.c2) % 32 == 0) {
+        if (wgs(16) >= 1024) return launch_conv_lds<16>(a, st);
+        if (wgs(8) >= 1024) return launch_conv_lds<8>(a, st);
+        if (wgs(4) >= 64) return launch_conv_lds<4>(a, st);
+    }
+    return launch_conv<false>(a, st);
+}
 hipError_t conv3x3_split(const ConvArgs& a, hipStream_t st) { return launch_conv<true>(a, st); }
 
 hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, int mode1, const bf16* dz, int cout,
