@@ -480,7 +480,12 @@ struct mpm_surrogate {
     float* wm = nullptr;
     float* target = nullptr;
     int* slots_d = nullptr;
+    int* live_slot_d = nullptr;     // the live slot's index, for k_pack
     float* loss_d = nullptr;
+    SlotInfo* info_h = nullptr;     // pinned copy of the live slot's info
+    hipEvent_t ev_solver = nullptr; // orders a capture after the solver's work
+    hipEvent_t ev_capture = nullptr;// orders the solver's next work after a capture
+    hipEvent_t ev_info = nullptr;   // the live slot's info has reached info_h
     int live() const { return capacity; }
 };
 
@@ -518,8 +523,10 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     int r = check_scene(g, s, scene);
     if (r) return r;
     if ((r = ensure_xs(g, s))) return r;
-    // the solver's work is on its own stream: order the capture after it
-    S_TRY(hipStreamSynchronize(s->stream));
+    // the solver's work is on its own stream: the capture waits for it, and the
+    // solver's next work waits for the capture (events, no host synchronisation)
+    S_TRY(hipEventRecord(g->ev_solver, s->stream));
+    S_TRY(hipStreamWaitEvent(g->st, g->ev_solver, 0));
     const mpm_params& P = s->scene_params[scene];
     const Phys ph = s->scenes_h[scene].ph;
     const int hw = g->h * g->w;
@@ -543,6 +550,8 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     k_capture_finish<<<nbg, 256, 0, g->st>>>(sample, hw, g->part_g, mode);
     if (mode == 0) k_info_reduce<<<1, 256, 0, g->st>>>(g->part_p, nbp, g->part_g, nbg, info);
     S_TRY(hipGetLastError());
+    S_TRY(hipEventRecord(g->ev_capture, g->st));
+    S_TRY(hipStreamWaitEvent(s->stream, g->ev_capture, 0));
     return MPM_OK;
 }
 
@@ -588,9 +597,16 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
         hipMalloc(&g->target, (size_t)max_batch * hw * 2 * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->slots_d, (size_t)max_batch * sizeof(int)) != hipSuccess ||
         hipMalloc(&g->loss_d, 256) != hipSuccess ||
-        hipMalloc(&g->part_g, (hw / 256 + 1) * sizeof(float)) != hipSuccess)
+        hipMalloc(&g->part_g, (hw / 256 + 1) * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->live_slot_d, sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&g->info_h, sizeof(SlotInfo), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_solver, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_capture, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_info, hipEventDisableTiming) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "surrogate allocation failed"));
-    if (hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
+    const int live_slot = capacity;
+    if (hipMemcpyAsync(g->live_slot_d, &live_slot, sizeof(int), hipMemcpyHostToDevice, g->st) != hipSuccess ||
+        hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
         hipMemsetAsync(g->in, 0, (size_t)max_batch * hw * 32 * sizeof(bf16), g->st) != hipSuccess ||
         hipMemsetAsync(g->wm, 0, (size_t)max_batch * hw * sizeof(float), g->st) != hipSuccess ||
         hipMemsetAsync(g->target, 0, (size_t)max_batch * hw * 2 * sizeof(float), g->st) != hipSuccess ||
@@ -614,7 +630,12 @@ int mpm_surrogate_destroy(mpm_surrogate* g)
     (void)hipFree(g->wm);
     (void)hipFree(g->target);
     (void)hipFree(g->slots_d);
+    (void)hipFree(g->live_slot_d);
     (void)hipFree(g->loss_d);
+    if (g->info_h) (void)hipHostFree(g->info_h);
+    if (g->ev_solver) (void)hipEventDestroy(g->ev_solver);
+    if (g->ev_capture) (void)hipEventDestroy(g->ev_capture);
+    if (g->ev_info) (void)hipEventDestroy(g->ev_info);
     if (g->st) (void)hipStreamDestroy(g->st);
     delete g;
     return MPM_OK;
@@ -787,35 +808,38 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     if (mode == MPM_ADVANCE_PLAIN || s->n == 0) return mpm_step(s, k_substeps);
     int r = capture(g, s, 0, g->live(), 0, k_substeps);
     if (r) return r;
-    SlotInfo I{};
-    S_TRY(hipMemcpyAsync(&I, g->info + g->live(), sizeof(I), hipMemcpyDeviceToHost, g->st));
-    S_TRY(hipStreamSynchronize(g->st));
+    const int hw = g->h * g->w;
+    S_TRY(hipMemcpyAsync(g->info_h, g->info + g->live(), sizeof(SlotInfo), hipMemcpyDeviceToHost, g->st));
+    S_TRY(hipEventRecord(g->ev_info, g->st));
+    // the network runs while the host waits for the residual test's inputs (it reads
+    // only the live slot; if the test fails its output is not used)
+    const float* out = nullptr;
+    if (mode == MPM_ADVANCE_NETWORK) {
+        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->live_slot_d, 1, hw,
+                                                                    g->d_ref, g->in, g->wm, nullptr);
+        S_TRY(hipGetLastError());
+        S_TRY(g->net.forward(1, g->in, g->wm));
+        out = g->net.out();
+    }
+    S_TRY(hipEventSynchronize(g->ev_info));
+    const SlotInfo I = *g->info_h;
     // share of mass that can exchange momentum with the walls in the next k
     // substeps, and mass outside the image: the prediction conserves momentum up to
     // the gravity impulse, so these bound its momentum and mass residuals
     const float M = I.particle_mass > 0.f ? I.particle_mass : 1.f;
     const float res = std::max(I.band_mass / M, std::fabs(M - I.grid_mass) / M);
     if (residual) *residual = res;
-    if (!(res <= tol)) return mpm_step(s, k_substeps);
+    if (!(res <= tol)) return mpm_step(s, k_substeps);   // after the capture (event), on the solver's stream
 
-    const int hw = g->h * g->w;
-    const float* out = nullptr;
-    if (mode == MPM_ADVANCE_NETWORK) {
-        const int live = g->live();
-        S_TRY(hipMemcpyAsync(g->slots_d, &live, sizeof(int), hipMemcpyHostToDevice, g->st));
-        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, 1, hw, g->d_ref,
-                                                                    g->in, g->wm, nullptr);
-        S_TRY(hipGetLastError());
-        S_TRY(g->net.forward(1, g->in, g->wm));
-        out = g->net.out();
-    }
     const Phys ph = s->scenes_h[0].ph;
     const float T = (float)k_substeps * ph.dt;
     const float* sample = g->data + (size_t)g->live() * NCH * hw;
     k_macro_g2p<<<(s->n + 255) / 256, 256, 0, g->st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
                                                        g->d_ref, T);
     S_TRY(hipGetLastError());
-    S_TRY(hipStreamSynchronize(g->st));
+    // the solver's next work waits for the macro step (no host synchronisation)
+    S_TRY(hipEventRecord(g->ev_capture, g->st));
+    S_TRY(hipStreamWaitEvent(s->stream, g->ev_capture, 0));
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;
     s->items_valid = false;
