@@ -518,15 +518,21 @@ int ensure_xs(mpm_surrogate* g, mpm_solver* s)
 }
 
 // Capture into `slot`: mode 0 the features (and saved positions), mode 1 the target.
-int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int k_substeps)
+// On the solver's own stream (on_solver_stream: a macro step's capture, ordered with
+// the solver's work by the stream itself), or on the surrogate's stream with events
+// ordering it after the solver's work and the solver's next work after it (no host
+// synchronisation either way).
+int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int k_substeps,
+            bool on_solver_stream = false)
 {
     int r = check_scene(g, s, scene);
     if (r) return r;
     if ((r = ensure_xs(g, s))) return r;
-    // the solver's work is on its own stream: the capture waits for it, and the
-    // solver's next work waits for the capture (events, no host synchronisation)
-    S_TRY(hipEventRecord(g->ev_solver, s->stream));
-    S_TRY(hipStreamWaitEvent(g->st, g->ev_solver, 0));
+    hipStream_t st = on_solver_stream ? s->stream : g->st;
+    if (!on_solver_stream) {
+        S_TRY(hipEventRecord(g->ev_solver, s->stream));
+        S_TRY(hipStreamWaitEvent(g->st, g->ev_solver, 0));
+    }
     const mpm_params& P = s->scene_params[scene];
     const Phys ph = s->scenes_h[scene].ph;
     const int hw = g->h * g->w;
@@ -534,24 +540,26 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     SlotInfo* info = g->info + slot;
     const int first = (int)s->p_off[scene], count = (int)(s->p_off[scene + 1] - s->p_off[scene]);
     if (mode == 0) {
-        k_zero<<<std::min((NCH * hw + 255) / 256, 4096), 256, 0, g->st>>>(sample, (int64_t)NCH * hw);
-        k_init_info<<<1, 1, 0, g->st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
+        k_zero<<<std::min((NCH * hw + 255) / 256, 4096), 256, 0, st>>>(sample, (int64_t)NCH * hw);
+        k_init_info<<<1, 1, 0, st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
     }
     const float band = P.BORDER_MARGIN + 2.0f;
     const int nbp = (count + 255) / 256, nbg = (hw + 255) / 256;
     if (count > 0) {
         if (mode == 0)
-            k_capture<0><<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
+            k_capture<0><<<nbp, 256, 0, st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
                                                  g->xs, g->part_p);
         else
-            k_capture<1><<<nbp, 256, 0, g->st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
+            k_capture<1><<<nbp, 256, 0, st>>>(s->dev_particles(), first, count, ph, g->h, g->w, band, sample,
                                                  g->xs, g->part_p);
     }
-    k_capture_finish<<<nbg, 256, 0, g->st>>>(sample, hw, g->part_g, mode);
-    if (mode == 0) k_info_reduce<<<1, 256, 0, g->st>>>(g->part_p, nbp, g->part_g, nbg, info);
+    k_capture_finish<<<nbg, 256, 0, st>>>(sample, hw, g->part_g, mode);
+    if (mode == 0) k_info_reduce<<<1, 256, 0, st>>>(g->part_p, nbp, g->part_g, nbg, info);
     S_TRY(hipGetLastError());
-    S_TRY(hipEventRecord(g->ev_capture, g->st));
-    S_TRY(hipStreamWaitEvent(s->stream, g->ev_capture, 0));
+    if (!on_solver_stream) {
+        S_TRY(hipEventRecord(g->ev_capture, g->st));
+        S_TRY(hipStreamWaitEvent(s->stream, g->ev_capture, 0));
+    }
     return MPM_OK;
 }
 
@@ -806,19 +814,25 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     if (used) *used = MPM_ADVANCE_PLAIN;
     if (residual) *residual = 0.f;
     if (mode == MPM_ADVANCE_PLAIN || s->n == 0) return mpm_step(s, k_substeps);
-    int r = capture(g, s, 0, g->live(), 0, k_substeps);
+    // the whole macro step runs on the solver's stream, in order with its work; the
+    // solver's stream first waits for the surrogate stream's pending work (training,
+    // recording), which shares the network's and the capture's buffers
+    hipStream_t st = s->stream;
+    S_TRY(hipEventRecord(g->ev_solver, g->st));
+    S_TRY(hipStreamWaitEvent(st, g->ev_solver, 0));
+    int r = capture(g, s, 0, g->live(), 0, k_substeps, true);
     if (r) return r;
     const int hw = g->h * g->w;
-    S_TRY(hipMemcpyAsync(g->info_h, g->info + g->live(), sizeof(SlotInfo), hipMemcpyDeviceToHost, g->st));
-    S_TRY(hipEventRecord(g->ev_info, g->st));
+    S_TRY(hipMemcpyAsync(g->info_h, g->info + g->live(), sizeof(SlotInfo), hipMemcpyDeviceToHost, st));
+    S_TRY(hipEventRecord(g->ev_info, st));
     // the network runs while the host waits for the residual test's inputs (it reads
     // only the live slot; if the test fails its output is not used)
     const float* out = nullptr;
     if (mode == MPM_ADVANCE_NETWORK) {
-        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->live_slot_d, 1, hw,
-                                                                    g->d_ref, g->in, g->wm, nullptr);
+        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, st>>>(g->data, g->info, g->live_slot_d, 1, hw, g->d_ref,
+                                                                 g->in, g->wm, nullptr);
         S_TRY(hipGetLastError());
-        S_TRY(g->net.forward(1, g->in, g->wm));
+        S_TRY(g->net.forward(1, g->in, g->wm, st));
         out = g->net.out();
     }
     S_TRY(hipEventSynchronize(g->ev_info));
@@ -829,17 +843,18 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     const float M = I.particle_mass > 0.f ? I.particle_mass : 1.f;
     const float res = std::max(I.band_mass / M, std::fabs(M - I.grid_mass) / M);
     if (residual) *residual = res;
-    if (!(res <= tol)) return mpm_step(s, k_substeps);   // after the capture (event), on the solver's stream
+    // later work on the surrogate's stream waits for this macro step's use of the
+    // shared buffers
+    S_TRY(hipEventRecord(g->ev_capture, st));
+    S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
+    if (!(res <= tol)) return mpm_step(s, k_substeps);
 
     const Phys ph = s->scenes_h[0].ph;
     const float T = (float)k_substeps * ph.dt;
     const float* sample = g->data + (size_t)g->live() * NCH * hw;
-    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, g->st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
-                                                       g->d_ref, T);
+    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
+                                                    g->d_ref, T);
     S_TRY(hipGetLastError());
-    // the solver's next work waits for the macro step (no host synchronisation)
-    S_TRY(hipEventRecord(g->ev_capture, g->st));
-    S_TRY(hipStreamWaitEvent(s->stream, g->ev_capture, 0));
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;
     s->items_valid = false;

@@ -133,9 +133,10 @@ hipError_t UNet::sync_weights()
     return hipSuccess;
 }
 
-hipError_t UNet::forward(int nb, const bf16* x, const float* wm)
+hipError_t UNet::forward(int nb, const bf16* x, const float* wm, hipStream_t stream)
 {
     const int H = h_, W = w_;
+    hipStream_t st_ = stream ? stream : this->st_;
     // x1 of e1..e3 is the pooled output of the level above, x1 of d2..d0 the
     // upsampled output of the level below: the convolutions read them in place
     auto conv = [&](int l, const bf16* x1, int mode1, const bf16* x2, int hh, int ww, bf16* z, bf16* a) {

@@ -51,8 +51,9 @@ public:
     // Refresh the BF16 weight copies after the FP32 parameters changed.
     hipError_t sync_weights();
     // Forward pass: x [nb][h][w][32] BF16, wm [nb][h*w] projection weights; the
-    // projected output (2 channels, FP32) is left in out().
-    hipError_t forward(int nb, const unet::bf16* x, const float* wm);
+    // projected output (2 channels, FP32) is left in out().  On `stream` when given,
+    // else on the network's own stream.
+    hipError_t forward(int nb, const unet::bf16* x, const float* wm, hipStream_t stream = nullptr);
     // Loss of the projected output against target [nb][h*w][2] into *loss (device
     // scalar, accumulated); with backward, the parameter gradient into grads() (zeroed first).
     hipError_t loss_and_backward(int nb, const unet::bf16* x, const float* wm, const float* target, float* loss,
