@@ -204,6 +204,12 @@ int mpm_surrogate_get_params(mpm_surrogate* g, float* host);
 int mpm_surrogate_set_params(mpm_surrogate* g, const float* host);
 /* Velocity scale of the network output (m/s; default 0.1). */
 int mpm_surrogate_set_scale(mpm_surrogate* g, float d_ref);
+/* Kernel of the forward 3x3 convolutions (no reference counterpart: the surrogate
+ * is this library's own): MPM_CONV_AUTO (default) chooses by layer shape,
+ * MPM_CONV_STREAM the register-streaming kernel, MPM_CONV_LDS the LDS-tiled one.
+ * Results agree up to the order of FP32 accumulation. */
+enum { MPM_CONV_AUTO = 0, MPM_CONV_STREAM = 1, MPM_CONV_LDS = 2 };
+int mpm_surrogate_set_conv(mpm_surrogate* g, int kernel);
 /* Network on host data: input [nb][h][w][32] BF16 bit patterns, wm [nb][h*w]
  * projection weights (summing to one per image), target [nb][h*w][2].  forward
  * returns the projected output [nb][h*w][2]; gradient returns the loss

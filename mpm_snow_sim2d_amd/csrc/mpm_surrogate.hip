@@ -673,6 +673,14 @@ int mpm_surrogate_set_params(mpm_surrogate* g, const float* host)
     return MPM_OK;
 }
 
+int mpm_surrogate_set_conv(mpm_surrogate* g, int kernel)
+{
+    if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    if (kernel < MPM_CONV_AUTO || kernel > MPM_CONV_LDS) return sfail(MPM_E_INVALID, "unknown convolution kernel");
+    g->net.set_conv_kernel(kernel);
+    return MPM_OK;
+}
+
 int mpm_surrogate_set_scale(mpm_surrogate* g, float d_ref)
 {
     if (!g) return sfail(MPM_E_INVALID, "NULL argument");

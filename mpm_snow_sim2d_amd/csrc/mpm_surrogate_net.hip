@@ -146,7 +146,7 @@ hipError_t UNet::forward(int nb, const bf16* x, const float* wm, hipStream_t str
         c.nb = nb; c.h = hh; c.w = ww;
         c.wt = wf_[l]; c.bias = params_ + L.b_off; c.cout = L.cout;
         c.z = z; c.a = a;
-        return unet::conv3x3_forward(c, st_);
+        return unet::conv3x3_forward(c, st_, conv_kernel_);
     };
     NET_TRY(conv(0, x, 0, nullptr, H, W, z0_, a0_));
     NET_TRY(conv(1, a0_, 2, nullptr, H / 2, W / 2, z1_, a1_));

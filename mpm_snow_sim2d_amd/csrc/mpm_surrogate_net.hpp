@@ -60,9 +60,12 @@ public:
                                  bool backward);
     hipError_t adam(float lr, float beta1 = 0.9f, float beta2 = 0.999f, float eps = 1e-8f);
     float* out() { return out_; }
+    // forward convolution kernel: 0 by layer shape, 1 register-streaming, 2 LDS-tiled
+    void set_conv_kernel(int k) { conv_kernel_ = k; }
 
 private:
     int h_ = 0, w_ = 0, max_batch_ = 0;
+    int conv_kernel_ = 0;
     hipStream_t st_ = nullptr;
     UNetLayer layers_[NLAYERS] = {};
     size_t n_params_ = 0, head_w_off_ = 0, head_b_off_ = 0;

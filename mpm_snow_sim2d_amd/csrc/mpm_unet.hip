@@ -845,14 +845,14 @@ inline unsigned head_blocks(int hw)
 // their staging and matrix work overlap), 4-row tiles for smaller layers down to
 // 64 workgroups (measured faster than the register-streaming kernel there), and
 // the register-streaming kernel with its split reduction below that.
-hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st)
+hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st, int kernel)
 {
     if (a.nb * a.h * a.w == 0) return hipSuccess;
     auto wgs = [&](int th) { return (long long)a.nb * ((a.h + th - 1) / th) * ((a.w + 15) / 16) * (a.cout / 32); };
-    if (a.cout % 32 == 0 && (a.c1 + a.c2) % 32 == 0) {
+    if (kernel != 1 && a.cout % 32 == 0 && (a.c1 + a.c2) % 32 == 0) {
         if (wgs(16) >= 1024) return launch_conv_lds<16>(a, st);
         if (wgs(8) >= 1024) return launch_conv_lds<8>(a, st);
-        if (wgs(4) >= 64) return launch_conv_lds<4>(a, st);
+        if (wgs(4) >= 64 || kernel == 2) return launch_conv_lds<4>(a, st);
     }
     return launch_conv<false>(a, st);
 }

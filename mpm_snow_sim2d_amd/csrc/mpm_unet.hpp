@@ -36,7 +36,8 @@ struct ConvArgs {
     int split;
     int add1, add2;
 };
-hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st);
+// kernel: 0 chooses by layer shape, 1 the register-streaming kernel, 2 the LDS-tiled one
+hipError_t conv3x3_forward(const ConvArgs& a, hipStream_t st, int kernel = 0);
 hipError_t conv3x3_split(const ConvArgs& a, hipStream_t st);
 
 // dW[tap][co][ci] += sum over pixels of dz[p][co] * x[p shifted by tap][ci]  (FP32, atomics);

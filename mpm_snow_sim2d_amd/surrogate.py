@@ -21,6 +21,7 @@ from ._lib import check
 ADVANCE_PLAIN = _lib.ADVANCE_PLAIN
 ADVANCE_NETWORK = _lib.ADVANCE_NETWORK
 ADVANCE_BALLISTIC = _lib.ADVANCE_BALLISTIC
+CONV_AUTO, CONV_STREAM, CONV_LDS = _lib.CONV_AUTO, _lib.CONV_STREAM, _lib.CONV_LDS
 
 
 def _ptr(a):
@@ -78,6 +79,11 @@ class Surrogate:
     def set_scale(self, d_ref: float) -> None:
         self.d_ref = float(d_ref)
         check(self._lib.mpm_surrogate_set_scale(self._h, ctypes.c_float(d_ref)))
+
+    def set_conv(self, kernel: int) -> None:
+        """Forward convolution kernel: CONV_AUTO (by layer shape), CONV_STREAM
+        (register-streaming) or CONV_LDS (LDS-tiled)."""
+        check(self._lib.mpm_surrogate_set_conv(self._h, int(kernel)))
 
     # -- the network on host arrays
     def forward(self, input_bits: np.ndarray, wm: np.ndarray) -> np.ndarray:

@@ -68,6 +68,58 @@ def test_network_forward_matches_oracle():
     assert err < 1e-2
 
 
+@pytest.mark.parametrize("kernel", ["stream", "lds"])
+def test_network_forward_batch_matches_oracle(kernel):
+    """16 images of 32x32: with the LDS-tiled convolutions (4-row tiles) and with
+    the register-streaming ones, against the FP64 oracle."""
+    from mpm_snow_sim2d_amd.surrogate import CONV_LDS, CONV_STREAM, Surrogate
+    rng = np.random.default_rng(5)
+    nb, h, w = 16, 32, 32
+    _flush(f"[surrogate] forward {nb}x{h}x{w} {kernel}")
+    with Surrogate(h, w, max_batch=nb, seed=4) as sur:
+        sur.set_conv(CONV_LDS if kernel == "lds" else CONV_STREAM)
+        prm = sur.get_params()
+        o, _ = U.layout()[0]["head.w"]
+        prm[o:o + 64] = rng.normal(scale=0.5, size=64)
+        sur.set_params(prm)
+        p, bits, xr, wm, _ = _net_problem(rng, sur, nb, h, w)
+        out = sur.forward(bits, wm)
+    ref = U.forward(p, xr, wm.astype(np.float64))
+    err = _rel(out, ref)
+    _flush(f"    rel L2 {err:.3e}")
+    assert err < 1e-2
+
+
+@pytest.mark.parametrize("nb,size", [(1, 256), (8, 128), (8, 256)])
+def test_lds_convolutions_match_streaming(nb, size):
+    """The LDS-tiled forward convolutions at the shapes that select each tile height
+    (4 rows: one 256^2 image; 8 rows: 8 images of 128^2; 16 rows: 8 images of
+    256^2) against the register-streaming kernels (checked against the oracle at
+    small sizes): equal up to FP32 accumulation order and the BF16 roundings of
+    the activations that order can flip."""
+    from mpm_snow_sim2d_amd.surrogate import CONV_AUTO, CONV_STREAM, Surrogate
+    rng = np.random.default_rng(6)
+    _flush(f"[surrogate] lds vs streaming {nb}x{size}x{size}")
+    with Surrogate(size, size, max_batch=nb, seed=5) as sur:
+        prm = sur.get_params()
+        o, _ = U.layout()[0]["head.w"]
+        prm[o:o + 64] = rng.normal(scale=0.5, size=64)
+        sur.set_params(prm)
+        x = rng.normal(size=(nb, size, size, 32)).astype(np.float32)
+        x[..., 8:] = 0.0
+        bits = U.to_bf16_bits(x).reshape(nb, size, size, 32)
+        m = rng.uniform(0.0, 1.0, size=(nb, size * size))
+        wm = (m / m.sum(1, keepdims=True)).astype(np.float32)
+        sur.set_conv(CONV_STREAM)
+        ref = sur.forward(bits, wm)
+        sur.set_conv(CONV_AUTO)
+        outs = [sur.forward(bits, wm) for _ in range(2)]
+    assert np.array_equal(outs[0], outs[1])   # no atomics: the LDS path is deterministic
+    err = _rel(outs[0], ref)
+    _flush(f"    rel L2 {err:.3e}")
+    assert err < 5e-3
+
+
 def test_network_gradient_matches_oracle():
     from mpm_snow_sim2d_amd.surrogate import Surrogate
     rng = np.random.default_rng(1)
