@@ -135,9 +135,14 @@ def learned(args) -> dict:
     flops = sur.flops(1)
 
     res = {ADVANCE_PLAIN: [], ADVANCE_NETWORK: [], ADVANCE_BALLISTIC: []}
+    floor = []
     for i, (Pq, ps) in enumerate(test):
         ref = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN)
         res[ADVANCE_PLAIN].append({"wall": ref["wall"], "err": 0.0, "counts": ref["counts"]})
+        # the same plain rollout binned every 16 substeps instead of 8: only the order
+        # of the float additions differs, so this is the divergence floor of the scene
+        ref2 = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN, options=Options(sort_every=16, tile_particles=256))
+        floor.append(float(np.linalg.norm(ref2["x"] - ref["x"], axis=1).mean()))
         for mode in (ADVANCE_NETWORK, ADVANCE_BALLISTIC):
             r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol)
             err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean())
@@ -180,6 +185,7 @@ def learned(args) -> dict:
             "mean_pos_err_m": err_net, "mean_pos_err_pct_domain": round(100 * err_net, 4),
             "within_1pct": bool(err_net <= 0.01),
             "ballistic_pos_err_m": agg(ADVANCE_BALLISTIC, "err"),
+            "plain_vs_plain_pos_err_m": float(np.mean(floor)),
             "network_macro_steps_share": round(net_steps / max(all_steps, 1), 4),
             "forward_ms": round(fwd_ms, 4), "train_step_ms": round(step_ms, 4),
         },
