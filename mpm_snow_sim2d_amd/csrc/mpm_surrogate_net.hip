@@ -93,6 +93,7 @@ int UNet::init(int h, int w, int max_batch, uint64_t seed, hipStream_t st, std::
     dproj_ = (float*)alloc(P0 * 2 * 4, &e);
     part_ = (float*)alloc(B * 64 * 2 * 4, &e);
     part2_ = (float*)alloc(B * 64 * 2 * 4, &e);
+    hpart_ = (float*)alloc(B * 64 * 66 * 4, &e);
     if (e != hipSuccess) {
         *err = std::string("U-Net allocation: ") + hipGetErrorString(e);
         release();
@@ -171,7 +172,7 @@ hipError_t UNet::loss_and_backward(int nb, const bf16* x, const float* wm, const
 
     NET_TRY(unet::fill_f32(grads_, (int64_t)n_params_, 0.f, st_));
     NET_TRY(unet::head_backward(b0_, dproj_, part2_, wm, nb, H * W, params_ + head_w_off_, g_b0_,
-                                grads_ + head_w_off_, grads_ + head_b_off_, st_));
+                                grads_ + head_w_off_, grads_ + head_b_off_, hpart_, st_));
 
     auto wgrad = [&](int l, const bf16* x1, int mode1, const bf16* x2, const bf16* dz, int hh,
                      int ww) -> hipError_t {

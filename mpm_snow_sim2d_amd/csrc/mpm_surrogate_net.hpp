@@ -85,6 +85,7 @@ private:
     float* dproj_ = nullptr;
     float* part_ = nullptr;    // [max_batch][64][2] partial projection sums
     float* part2_ = nullptr;   // [max_batch][64][2] partial gradient sums
+    float* hpart_ = nullptr;   // [max_batch][64][66] head gradient partials
     std::vector<void*> allocs_;
     void* alloc(size_t bytes, hipError_t* e);
 };

@@ -509,16 +509,26 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(const bf16* __restrict__ 
 __global__ void __launch_bounds__(256) k_bias_grad(const bf16* __restrict__ dz, int npix, int cout,
                                                   float* __restrict__ db)
 {
-    __shared__ float part[256];
-    const int c = threadIdx.x % cout, lanes = 256 / cout, k = threadIdx.x / cout;
-    const int p0 = blockIdx.x * 1024;
-    float s = 0.f;
-    for (int p = p0 + k; p < min(p0 + 1024, npix); p += lanes) s += (float)dz[(size_t)p * cout + c];
-    part[threadIdx.x] = s;
+    // thread: 8 consecutive channels (one 16-byte load per pixel) of every
+    // (256 / (cout/8))-th pixel of the workgroup's chunks; partial sums through LDS
+    __shared__ float part[256 * 8];
+    const int ngrp = cout >> 3, lanes = 256 / ngrp;
+    const int cg = threadIdx.x % ngrp, k = threadIdx.x / ngrp;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (k < lanes)
+        for (int p0 = blockIdx.x * 1024; p0 < npix; p0 += gridDim.x * 1024)
+            for (int p = p0 + k; p < min(p0 + 1024, npix); p += lanes) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(dz + (size_t)p * cout + 8 * cg);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s[j] += (float)v[j];
+            }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) part[threadIdx.x * 8 + j] = s[j];
     __syncthreads();
     if (threadIdx.x < cout) {
+        const int g = threadIdx.x >> 3, j = threadIdx.x & 7;
         float t = 0.f;
-        for (int j = 0; j < lanes; ++j) t += part[j * cout + threadIdx.x];
+        for (int kk = 0; kk < lanes; ++kk) t += part[(kk * ngrp + g) * 8 + j];
         atomicAdd(db + threadIdx.x, t);
     }
 }
@@ -650,6 +660,7 @@ __global__ void __launch_bounds__(256) k_gelu_bwd(const bf16* __restrict__ da, c
 // ---------------------------------------------------------------------------
 // Head (32 -> 2 channels, FP32) and the per-image momentum projection.
 constexpr int HEAD_C = 32;
+constexpr int HEAD_PART = 2 * HEAD_C + 2;   // head gradient values per partial row
 
 __device__ __forceinline__ float block_sum(float v, float* sh)
 {
@@ -754,9 +765,9 @@ __global__ void __launch_bounds__(256) k_head_backward(const bf16* __restrict__ 
                                                       const float* __restrict__ part2, int nparts,
                                                       const float* __restrict__ wm, int nb, int hw,
                                                       const float* __restrict__ wh, bf16* __restrict__ da,
-                                                      float* __restrict__ dwh, float* __restrict__ dbh)
+                                                      float* __restrict__ hpart)
 {
-    __shared__ float sh[4];
+    __shared__ float red[4][HEAD_PART];
     const int b = blockIdx.y;
     float t0 = 0.f, t1 = 0.f;
     for (int j = 0; j < nparts; ++j) {
@@ -789,16 +800,36 @@ __global__ void __launch_bounds__(256) k_head_backward(const bf16* __restrict__ 
             *reinterpret_cast<bf16x8*>(dr + 8 * k) = o;
         }
     }
+    // the workgroup's sums of the 2*HEAD_C weight and 2 bias gradients: per wave by
+    // shuffles, across the four waves through LDS, then to this workgroup's row of
+    // hpart (reduced by k_head_reduce: no same-address global atomics)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 2 * HEAD_C; ++k) {
-        const float t = block_sum(gw[k], sh);
-        if (threadIdx.x == 0) atomicAdd(dwh + k, t);
+    for (int k = 0; k < HEAD_PART; ++k) {
+        float v = k < 2 * HEAD_C ? gw[k] : (k == 2 * HEAD_C ? gb0 : gb1);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wave][k] = v;
     }
-    const float s0 = block_sum(gb0, sh);
-    const float s1 = block_sum(gb1, sh);
+    __syncthreads();
+    if (threadIdx.x < HEAD_PART)
+        hpart[((size_t)b * gridDim.x + blockIdx.x) * HEAD_PART + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// dwh (2*HEAD_C values) and dbh (2) accumulate the sum of the nrows partial rows:
+// workgroup k sums column k.
+__global__ void __launch_bounds__(256) k_head_reduce(const float* __restrict__ hpart, int nrows,
+                                                    float* __restrict__ dwh, float* __restrict__ dbh)
+{
+    __shared__ float sh[4];
+    const int k = blockIdx.x;
+    float v = 0.f;
+    for (int r = threadIdx.x; r < nrows; r += 256) v += hpart[(size_t)r * HEAD_PART + k];
+    v = block_sum(v, sh);
     if (threadIdx.x == 0) {
-        atomicAdd(dbh, s0);
-        atomicAdd(dbh + 1, s1);
+        if (k < 2 * HEAD_C) dwh[k] += v;
+        else dbh[k - 2 * HEAD_C] += v;
     }
 }
 
@@ -871,7 +902,7 @@ hipError_t conv3x3_wgrad(const bf16* x1, const bf16* x2, int c1, int c2, int mod
 hipError_t bias_grad(const bf16* dz, int npix, int cout, float* db, hipStream_t st)
 {
     if (npix == 0) return hipSuccess;
-    k_bias_grad<<<(npix + 1023) / 1024, 256, 0, st>>>(dz, npix, cout, db);
+    k_bias_grad<<<std::min((npix + 1023) / 1024, 256), 256, 0, st>>>(dz, npix, cout, db);
     return hipGetLastError();
 }
 
@@ -929,10 +960,11 @@ hipError_t head_project_loss(float* out, const float* part, const float* target,
 }
 
 hipError_t head_backward(const bf16* a, const float* dproj, const float* part2, const float* wm, int nb,
-                         int hw, const float* wh, bf16* da, float* dwh, float* dbh, hipStream_t st)
+                         int hw, const float* wh, bf16* da, float* dwh, float* dbh, float* hpart, hipStream_t st)
 {
     dim3 grid(head_blocks(hw), nb);
-    k_head_backward<<<grid, 256, 0, st>>>(a, dproj, part2, (int)head_blocks(hw), wm, nb, hw, wh, da, dwh, dbh);
+    k_head_backward<<<grid, 256, 0, st>>>(a, dproj, part2, (int)head_blocks(hw), wm, nb, hw, wh, da, hpart);
+    k_head_reduce<<<HEAD_PART, 256, 0, st>>>(hpart, (int)(head_blocks(hw) * nb), dwh, dbh);
     return hipGetLastError();
 }
 

@@ -72,8 +72,9 @@ hipError_t head_forward(const bf16* a, int nb, int hw, const float* wh, const fl
 hipError_t head_project_loss(float* out, const float* part, const float* target, const float* wm, int nb,
                              int hw, float* loss, float* dproj, float* part2, hipStream_t st);
 // Backward of projection and head: da (BF16, [p][32]), dwh/dbh accumulated (FP32).
+// hpart: scratch of nb * 64 rows of 66 floats (per-workgroup gradient partials).
 hipError_t head_backward(const bf16* a, const float* dproj, const float* part2, const float* wm, int nb,
-                         int hw, const float* wh, bf16* da, float* dwh, float* dbh, hipStream_t st);
+                         int hw, const float* wh, bf16* da, float* dwh, float* dbh, float* hpart, hipStream_t st);
 
 // Adam (FP32): m, v moments; lr_t includes the bias corrections.
 hipError_t adam_step(float* w, const float* g, float* m, float* v, int64_t n, float lr_t, float beta1,
