@@ -4,7 +4,9 @@ Workload (the configuration the headline metric is quoted on): a 0.3 m cube of
 2,000,376 seeded-jittered particles (126^3 lattice) dropped from 0.5 m, 256^3 grid,
 dx = 1/256, dt = 5e-5, quadratic MLS-MPM, sticky walls, Stomakhin snow parameters,
 FP32 state.  A step is one substep (grid update + fused G2P/P2G, plus the periodic
-binning).  Inputs are resident in HBM when the timed region starts.
+binning).  Inputs are resident in HBM when the timed region starts.  The binning
+period and work-item size come from the autotuner (short untimed trials on the
+same scene; --no-autotune uses --sort-every / --tile); neither changes results.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
   python bench.py --workload large|batch|snowballs2d [...]
@@ -287,7 +289,15 @@ def main() -> int:
         s = BatchSolver(scenes, opts)
         n = s.n
     else:
-        opts = Options(sort_every=args.sort_every, fused=1, tile_particles=args.tile)
+        if args.no_autotune:
+            opts = Options(sort_every=args.sort_every, fused=1, tile_particles=args.tile)
+        else:
+            # binning period and tile size from the autotuner on this scene (short
+            # trials before the run, not timed); results do not depend on either
+            opts, tuned_ms = autotune([(P, ps)], trial_substeps=64, graph=graph)
+            tuned = {"tile_particles": opts.tile_particles, "sort_every": opts.sort_every,
+                     "ms_per_substep": round(tuned_ms, 5)}
+            log(f"[bench] autotuned {tuned}")
         s = GPUSolver(P, ps, opts)
         n = ps.n
     log(f"[bench] {wl}: {n} particles ready")

@@ -170,7 +170,7 @@ int mpm_set_graph(mpm_solver* s, int enabled);
 
 /* Autotuner (no reference counterpart): times trial_substeps of the fused pipeline
  * (rounded up to whole binning periods) on fresh solvers built from the same input,
- * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32}, and
+ * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32, 64}, and
  * returns the fastest options and their time per substep.  nscenes = 1 tunes a
  * single scene (params/init point to one element).  The input is not modified. */
 int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,

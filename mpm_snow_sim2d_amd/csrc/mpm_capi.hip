@@ -622,7 +622,7 @@ int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* ini
     if (!best) return fail(MPM_E_INVALID, "best is NULL");
     if (trial_substeps < 1) return fail(MPM_E_INVALID, "trial_substeps must be >= 1");
     static const int tiles[] = {128, 256, 512};
-    static const int periods[] = {4, 8, 16, 32};
+    static const int periods[] = {4, 8, 16, 32, 64};
     double best_ms = -1.0;
     mpm_options chosen{};
     for (int tp : tiles)
