@@ -147,7 +147,7 @@ def test_autotune_picks_candidate_and_runs():
     best, ms = autotune(scenes, trial_substeps=16, graph=True)
     _flush(f"    best={best} ms/substep={ms:.4f}")
     assert best.tile_particles in (128, 256, 512)
-    assert best.sort_every in (4, 8, 16, 32)
+    assert best.sort_every in (4, 8, 16, 32, 64)
     assert best.fused == 1 and ms > 0
     with BatchSolver(scenes, best, graph=True) as b:
         b.Step(2 * best.sort_every + 1)
