@@ -207,10 +207,10 @@ def main() -> int:
     ap.add_argument("--sort-every", type=int, default=16)
     ap.add_argument("--n-side", type=int, default=126)
     ap.add_argument("--workload", choices=["cube", "large", "batch", "snowballs2d", "learned"], default="cube")
-    ap.add_argument("--train-scenes", type=int, default=32)
+    ap.add_argument("--train-scenes", type=int, default=96)
     ap.add_argument("--test-scenes", type=int, default=4)
-    ap.add_argument("--gen-substeps", type=int, default=800)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--gen-substeps", type=int, default=1600)
+    ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--rollout", type=int, default=2000)
