@@ -137,6 +137,10 @@ enum {
     MPM_KERNEL_COUNT = 6
 };
 int mpm_set_timing(mpm_solver* s, int enabled);
+/* Timing of the kernels whose bit (1 << MPM_KERNEL_*) is set only (0: off): fewer
+ * event records between launches when one kernel is measured.  No reference
+ * counterpart (instrumentation of this library's own kernels). */
+int mpm_set_timing_mask(mpm_solver* s, uint32_t kernels);
 int mpm_reset_timers(mpm_solver* s);
 int mpm_get_kernel_time(mpm_solver* s, int kernel, double* total_ms, int64_t* launches);
 

@@ -49,14 +49,15 @@ template <typename Fn>
 int timed(mpm_solver* s, int kernel, Fn&& fn)
 {
     hipEvent_t a = nullptr, b = nullptr;
-    if (s->timing) {
+    const bool on = s->timing && ((s->timing_mask >> kernel) & 1u);
+    if (on) {
         a = s->take_event();
         b = s->take_event();
         if (a) HIP_TRY(hipEventRecord(a, s->stream));
     }
     hipError_t e = fn();
     if (e != hipSuccess) return fail(MPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    if (s->timing && a && b) {
+    if (on && a && b) {
         HIP_TRY(hipEventRecord(b, s->stream));
         s->timers[kernel].pending.emplace_back(a, b);
     }
@@ -921,6 +922,15 @@ int mpm_set_timing(mpm_solver* s, int enabled)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     s->timing = enabled != 0;
+    s->timing_mask = ~0u;
+    return MPM_OK;
+}
+
+int mpm_set_timing_mask(mpm_solver* s, uint32_t kernels)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    s->timing = kernels != 0;
+    s->timing_mask = kernels;
     return MPM_OK;
 }
 

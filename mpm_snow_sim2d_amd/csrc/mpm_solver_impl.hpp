@@ -89,6 +89,7 @@ struct mpm_solver {
     int graph_grid = -1, graph_pb = -1, graph_period = 0;
 
     bool timing = false;
+    uint32_t timing_mask = ~0u;   // kernels timed while timing is on (bit MPM_KERNEL_*)
     mpm::detail::KernelTimer timers[MPM_KERNEL_COUNT];
     std::vector<hipEvent_t> event_pool;
 

@@ -162,8 +162,16 @@ class GPUSolver:
         return int(bad.value)
 
     # -- timing
-    def set_timing(self, on: bool = True) -> None:
-        check(self._lib.mpm_set_timing(self._h, int(on)))
+    def set_timing(self, on: bool = True, kernels=None) -> None:
+        """Per-kernel HIP-event timing; kernels: an iterable of KERNEL_* ids to time
+        only those (fewer event records between launches)."""
+        if kernels is None:
+            check(self._lib.mpm_set_timing(self._h, int(on)))
+        else:
+            mask = 0
+            for k in kernels:
+                mask |= 1 << int(k)
+            check(self._lib.mpm_set_timing_mask(self._h, mask if on else 0))
 
     def reset_timers(self) -> None:
         check(self._lib.mpm_reset_timers(self._h))
@@ -331,8 +339,16 @@ class BatchSolver:
                 None if Jp is None else _f32(Jp, (n,))]
         check(self._lib.mpm_set_state(self._h, *[_ptr(a) for a in arrs]))
 
-    def set_timing(self, on: bool = True) -> None:
-        check(self._lib.mpm_set_timing(self._h, int(on)))
+    def set_timing(self, on: bool = True, kernels=None) -> None:
+        """Per-kernel HIP-event timing; kernels: an iterable of KERNEL_* ids to time
+        only those (fewer event records between launches)."""
+        if kernels is None:
+            check(self._lib.mpm_set_timing(self._h, int(on)))
+        else:
+            mask = 0
+            for k in kernels:
+                mask |= 1 << int(k)
+            check(self._lib.mpm_set_timing_mask(self._h, mask if on else 0))
 
     def reset_timers(self) -> None:
         check(self._lib.mpm_reset_timers(self._h))
