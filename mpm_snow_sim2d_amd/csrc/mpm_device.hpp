@@ -99,12 +99,30 @@ __host__ __device__ inline int block_of(const DevGrid& g, int i, int j, int k)
 }
 
 // ---------------------------------------------------------------------------
+// Cell index of a position in grid units.  Within +-2^30 it is floor (or int()
+// truncation) exactly; larger magnitudes and infinities become +-2^30, far outside
+// any grid, so that an index plus a stencil or tile offset never overflows the int
+// range (a plain conversion saturates at INT_MAX, and INT_MAX + 1 wraps to a
+// negative index that bounds tests can take for an inside one); NaN gives 0, as the
+// hardware conversion does.
+__host__ __device__ inline int cell_floor(float v)
+{
+    const float f = floorf(v);
+    return f != f ? 0 : (int)fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+}
+
+__host__ __device__ inline int cell_trunc(float v)
+{
+    const float f = truncf(v);
+    return f != f ? 0 : (int)fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+}
+
 // B-spline weights along one axis (shaders/2d/common.glsl:43-55 for cubic).
 template <int ORDER> struct Spline;
 
 template <> struct Spline<3> {
     static constexpr int W = 4;
-    __device__ static inline int base(float xg) { return (int)xg - 1; }   // int() truncation
+    __host__ __device__ static inline int base(float xg) { return cell_trunc(xg) - 1; }   // int() truncation
     __device__ static inline float N(float x)
     {
         float a = fabsf(x);
@@ -124,7 +142,7 @@ template <> struct Spline<3> {
 
 template <> struct Spline<2> {
     static constexpr int W = 3;
-    __device__ static inline int base(float xg) { return (int)floorf(xg - 0.5f); }
+    __host__ __device__ static inline int base(float xg) { return cell_floor(xg - 0.5f); }
     __device__ static inline float N(float x)
     {
         float a = fabsf(x);

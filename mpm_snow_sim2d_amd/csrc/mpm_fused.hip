@@ -641,6 +641,15 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         // line as one request (global atomics are bound by requests, not lanes).
         const float inv_m = 1.f / scale_m, inv_p = 1.f / scale_p;
         constexpr int NC = DIM + 1;
+        // only nodes inside the grid receive tile adds; for a tile reaching past a
+        // wall each entry is tested as well, so that an entry outside the grid never
+        // addresses memory (the test is uniform, and skipped, for interior tiles)
+        bool tile_in = true;
+        {
+            const int lim[3] = {ph.W, ph.H, ph.D};
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) tile_in &= o[a] >= 0 && o[a] + T - 1 <= lim[a];
+        }
         for (int pe = tid; pe < TN * NC; pe += 256) {
             const int e = pe / NC, c = pe - e * NC;
             const int acc = s_acc[c][e];
@@ -648,6 +657,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             int t[3];
             tile_coords<DIM>(e, t);
             int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];
+            if (!tile_in && !in_grid<DIM>(ph, i, j, k)) continue;
             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, i, j, k)]) + c;
             unsafeAtomicAdd(dstp, (float)acc * (c == 0 ? inv_m : inv_p));
             int rbx = (i >> BlockShape<DIM>::SHIFT) - (bc[0] - 1);

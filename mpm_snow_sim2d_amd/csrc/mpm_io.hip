@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_splat(const float* __restrict__ x, int 
     // normalised device coordinates -> window pixels (y up, as glViewport)
     const float sx = (cx / cw * 0.5f + 0.5f) * (float)w;
     const float sy = (cy / cw * 0.5f + 0.5f) * (float)h;
-    const int ix = (int)floorf(sx), iy = (int)floorf(sy);
+    const int ix = cell_floor(sx), iy = cell_floor(sy);
     const int r0 = -(psize - 1) / 2, r1 = psize / 2;
     for (int dy = r0; dy <= r1; ++dy)
         for (int dx = r0; dx <= r1; ++dx) {

@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(256) k_cell_keys(const float* x, int n, float 
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         float xg = x[d * n + p] * inv_dx;
-        int ci = (int)floorf(xg);
+        int ci = cell_floor(xg);
         c[d] = min(max(ci, 0), lim[d]);
     }
     keys[p] = (uint32_t)node_addr<DIM>(g, c[0], c[1], c[2]);
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) k_cell_keys_batch(const float* x, int n, 
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         float xg = x[d * n + p] * sd.ph.inv_dx;
-        int ci = (int)floorf(xg);
+        int ci = cell_floor(xg);
         c[d] = min(max(ci, 0), lim[d]);
     }
     keys[p] = (uint32_t)(sd.block_off * 64 + node_addr<DIM>(g, c[0], c[1], c[2]));

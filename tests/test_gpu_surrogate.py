@@ -307,6 +307,7 @@ def test_trained_network_beats_ballistic():
             ref = rollout(None, Pq, ps, nsub, k, ADVANCE_PLAIN)["x"]
             for mode in errs:
                 r = rollout(sur, Pq, ps, nsub, k, mode, tol=1e-3)
+                _flush(f"    mode {mode}: macro steps by mode {r['counts']}")
                 assert r["nonfinite"] == 0
                 errs[mode].append(float(np.linalg.norm(r["x"] - ref, axis=1).mean()))
         net, bal = np.mean(errs[ADVANCE_NETWORK]), np.mean(errs[ADVANCE_BALLISTIC])
