@@ -212,3 +212,49 @@ def weight(x: float, order: int, dtype=np.float64) -> float:
 def dweight(x: float, order: int, dtype=np.float64) -> float:
     lib = load()
     return getattr(lib, f"orc_dweight_{_sfx(dtype)}")(x, order)
+
+
+OMP_LIB_PATH = os.path.join(HERE, "libmpm_oracle_omp.so")
+_omp = None
+
+
+def load_omp():
+    """The FP32 substep on every host core (OpenMP build of the same body): the
+    all-cores CPU baseline.  Not the checker — its grid sums add in thread order."""
+    global _omp
+    if _omp is not None:
+        return _omp
+    if not os.path.exists(OMP_LIB_PATH) or os.path.getmtime(OMP_LIB_PATH) < max(
+            os.path.getmtime(os.path.join(HERE, f)) for f in ("mpm_oracle_omp.c", "mpm_oracle_impl.inc")):
+        build()
+    lib = ctypes.CDLL(OMP_LIB_PATH)
+    Params = _params_type()
+    vp = ctypes.c_void_p
+    lib.orc_num_nodes.restype = ctypes.c_long
+    lib.orc_num_nodes.argtypes = [ctypes.POINTER(Params)]
+    lib.orc_omp_threads.restype = ctypes.c_int
+    lib.orc_omp_threads.argtypes = []
+    lib.orc_run_omp_f32.restype = None
+    lib.orc_run_omp_f32.argtypes = [ctypes.POINTER(Params), ctypes.c_long, ctypes.c_int] + [vp] * 11
+    _omp = lib
+    return lib
+
+
+def omp_threads() -> int:
+    return int(load_omp().orc_omp_threads())
+
+
+def run_omp(params, st: OracleState, nsteps: int) -> OracleState:
+    """nsteps FP32 substeps of the MLS scheme on every host core."""
+    if st.dtype != np.float32:
+        raise ValueError("the multi-threaded oracle runs in FP32")
+    if getattr(params, "scheme", 0) != 0:
+        raise ValueError("the multi-threaded oracle runs the MLS scheme only")
+    lib = load_omp()
+    G = lib.orc_num_nodes(ctypes.byref(params))
+    d = st.dim
+    gm = np.zeros(G, np.float32)
+    gv = [np.zeros(d * G, np.float32) for _ in range(3)]
+    lib.orc_run_omp_f32(ctypes.byref(params), st.n, int(nsteps), _p(st.x), _p(st.v), _p(st.mass), _p(st.vol),
+                        _p(st.B), _p(st.Fe), _p(st.Fp), _p(gm), _p(gv[0]), _p(gv[1]), _p(gv[2]))
+    return st

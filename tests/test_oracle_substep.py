@@ -269,3 +269,23 @@ def test_f32_oracle_tracks_f64(orc, dim):
     assert np.abs(s32.x - s64.x).max() < 1e-5
     assert np.abs(s32.v - s64.v).max() < 1e-3 * np.abs(s64.v).max()
     assert np.abs(s32.Fe - s64.Fe).max() < 1e-4
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_multithreaded_baseline_matches_serial_oracle(orc, dim):
+    """The all-cores CPU baseline (OpenMP build of the same body) runs the same
+    substep: after 10 substeps it agrees with the serial FP32 oracle up to the order
+    of the grid's float sums."""
+    P = snow_params(dim, 32, 1e-4)
+    rng = np.random.default_rng(6)
+    a = _block(dim, [0.3] * dim, [0.42] * dim, 0.012, [2.0, 0.0, 0.0][:dim], rng)
+    b = _block(dim, [0.44] + [0.32] * (dim - 1), [0.56] + [0.44] * (dim - 1), 0.012, [-2.0, 0.0, 0.0][:dim], rng)
+    ps = a.concat(b)
+    ser = orc.OracleState.from_particles(ps, np.float32)
+    par = orc.OracleState.from_particles(ps, np.float32)
+    orc.run(P, ser, 10)
+    orc.run_omp(P, par, 10)
+    assert orc.omp_threads() >= 1
+    assert np.abs(par.x - ser.x).max() < 1e-5
+    assert np.abs(par.v - ser.v).max() < 1e-3 * np.abs(ser.v).max()
+    assert np.abs(par.Fe - ser.Fe).max() < 1e-4
