@@ -24,8 +24,9 @@ pass) at 200 algorithmic bytes per particle in 3D, 104 in 2D (FP32 read + write
 of x, v, C, F_E, J_P) against the MI355X HBM peak; its duration comes from HIP
 events around each launch on the solver's stream over the timed region (only that
 kernel is instrumented there; the grid update and binning are timed in a further
-pass).  `cpu_baseline` times the FP32 CPU oracle (the same algorithm, one host
-thread) on 2 substeps of the same scene.
+pass).  `cpu_baseline` times the FP32 CPU oracle (the same algorithm) on the same
+scene: the OpenMP build on every host core (the reported value) and the serial
+oracle on one thread beside it, each for about 6 s of substeps.
 """
 from __future__ import annotations
 
@@ -61,18 +62,34 @@ def traffic_from_profiles() -> float | None:
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(params, ps, substeps: int = 2, what: str | None = None) -> dict:
+def cpu_baseline(params, ps, substeps: int = 1, what: str | None = None, budget_s: float = 6.0) -> dict:
+    """The FP32 CPU oracle (the same algorithm) on the host, each leg for as many
+    substeps as fit in about `budget_s` seconds after one untimed substep (at least
+    `substeps`): the serial oracle on one thread (the reference's CPU path is
+    serial), then the OpenMP build of the same body on every host core.  The
+    all-cores rate is the reported value."""
     from oracle import oracle as orc
-    st = orc.OracleState.from_particles(ps, np.float32)
+    scene = f"{what or 'the full scene'} ({ps.n} particles, grid {params.WIDTH}^{params.dim})"
+
+    def leg(run):
+        st = orc.OracleState.from_particles(ps, np.float32)
+        t0 = time.perf_counter()
+        run(params, st, 1)                          # first touch of the grid pages
+        per = time.perf_counter() - t0
+        k = max(substeps, min(4096, int(budget_s / max(per, 1e-6))))
+        t0 = time.perf_counter()
+        run(params, st, k)
+        dt = time.perf_counter() - t0
+        return ps.n * k / dt / 1e6, k, dt
+
     orc.load()
-    t0 = time.perf_counter()
-    orc.run(params, st, substeps)
-    dt = time.perf_counter() - t0
-    return {"value": ps.n * substeps / dt / 1e6, "unit": "M particle-substeps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"FP32 CPU oracle (GLSL-order restatement), 1 thread, {substeps} substeps of "
-                      f"{what or 'the full scene'} ({ps.n} particles, grid {params.WIDTH}^{params.dim}, "
-                      f"{dt:.1f} s)"}
+    one, k1, dt1 = leg(orc.run)
+    threads = orc.omp_threads()
+    many, kn, dtn = leg(orc.run_omp)
+    return {"value": many, "unit": "M particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 CPU oracle (GLSL-order restatement), OpenMP on {threads} threads, {kn} substeps of "
+                      f"{scene} in {dtn:.1f} s; the serial oracle, 1 thread, {k1} substeps in {dt1:.1f} s",
+            "single_thread": {"value": one, "cores": 1, "substeps": k1, "seconds": round(dt1, 2)}}
 
 
 def copy_calibration(gib: float = 2.0) -> None:
@@ -114,6 +131,8 @@ def learned(args) -> dict:
     test = [(P, colliding_scene(5000 + i, npart)) for i in range(nte)]
     log(f"[learned] scenes ready ({time.perf_counter() - t0:.1f} s)")
     sur = Surrogate(cells, cells, max_batch=args.batch, capacity=(ntr + nte) * per, seed=0, d_ref=0.1)
+    sur.set_macro(args.macro)
+    sur.set_targets(args.refine)
     t0 = time.perf_counter()
     tr = generate(sur, train, args.gen_substeps, k, wall_tol=args.tol)
     te = generate(sur, test, args.gen_substeps, k, first_slot=ntr * per, wall_tol=args.tol)
@@ -134,12 +153,20 @@ def learned(args) -> dict:
     log(f"[learned] trained {args.train_steps} steps in {train_s:.1f} s; held-out loss {val_loss:.4f} "
         f"(ballistic {base_loss:.4f})")
 
-    fwd_ms = sur.time(1, iters=20)
-    step_ms = sur.time(args.batch, iters=5, train=True)
-    flops = sur.flops(1)
+    # a macro step runs the network on 1 image (velocity), or 2 with the
+    # displacement; a training step on as many per slot
+    ni = 2 if args.macro else 1
+    fwd_ms = sur.time(ni, iters=20)
+    step_ms = sur.time(ni * args.batch, iters=5, train=True)
+    flops = sur.flops(ni)
 
     res = {ADVANCE_PLAIN: [], ADVANCE_NETWORK: [], ADVANCE_BALLISTIC: []}
     floor = []
+    # other macro forms of the same network: only a network trained with the
+    # displacement image (--macro > 0) can run the forms that use it
+    variants = [int(v) for v in args.macro_variants.split(",")
+                if v != "" and int(v) != args.macro and (args.macro or int(v) == 0)]
+    var_res = {fl: [] for fl in variants}
     for i, (Pq, ps) in enumerate(test):
         ref = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN)
         res[ADVANCE_PLAIN].append({"wall": ref["wall"], "err": 0.0, "counts": ref["counts"]})
@@ -148,9 +175,17 @@ def learned(args) -> dict:
         ref2 = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN, options=Options(sort_every=16, tile_particles=256))
         floor.append(float(np.linalg.norm(ref2["x"] - ref["x"], axis=1).mean()))
         for mode in (ADVANCE_NETWORK, ADVANCE_BALLISTIC):
+            sur.set_macro(args.macro)
             r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol)
             err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean())
             res[mode].append({"wall": r["wall"], "err": err, "counts": r["counts"], "bad": r["nonfinite"]})
+        # the same network with the other macro-step forms (which field moves the
+        # particles, which advances F_E)
+        for fl in variants:
+            sur.set_macro(fl)
+            r = rollout(sur, Pq, ps, args.rollout, k, ADVANCE_NETWORK, tol=args.tol)
+            var_res[fl].append(float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean()))
+        sur.set_macro(args.macro)
         log(f"[learned] scene {i}: plain {ref['wall'] * 1e3:.1f} ms; network {res[ADVANCE_NETWORK][-1]['wall'] * 1e3:.1f} ms "
             f"err {res[ADVANCE_NETWORK][-1]['err']:.2e} m uses {res[ADVANCE_NETWORK][-1]['counts']}; ballistic err "
             f"{res[ADVANCE_BALLISTIC][-1]['err']:.2e} m")
@@ -179,7 +214,9 @@ def learned(args) -> dict:
         "data": "synthetic",
         "config": {
             "workload": f"learned 2D surrogate: 256^2 grid, colliding scenes of ~{npart} particles (2-3 disks/boxes), "
-                        f"4-level BF16 U-Net 32-64-128-128, k=8 substeps per macro step, residual tolerance {args.tol}",
+                        f"4-level BF16 U-Net 32-64-128-128 predicting grid velocity"
+                        f"{' and displacement' if args.macro else ''}, target refinement {args.refine}, "
+                        f"k=8 substeps per macro step, residual tolerance {args.tol}",
             "train_scenes": ntr, "generated_substeps_per_scene": args.gen_substeps,
             "train_samples": len(tr_slots), "train_steps": args.train_steps, "batch": args.batch,
             "generate_s": round(gen_s, 2), "train_s": round(train_s, 2),
@@ -191,9 +228,11 @@ def learned(args) -> dict:
             "ballistic_pos_err_m": agg(ADVANCE_BALLISTIC, "err"),
             "plain_vs_plain_pos_err_m": float(np.mean(floor)),
             "network_macro_steps_share": round(net_steps / max(all_steps, 1), 4),
+            "macro_flags": args.macro, "target_refine_iters": args.refine,
+            "mean_pos_err_m_by_macro_flags": {str(fl): float(np.mean(v)) for fl, v in var_res.items()},
             "forward_ms": round(fwd_ms, 4), "train_step_ms": round(step_ms, 4),
         },
-        "roofline": {"bound": "mfma", "kernel": "U-Net forward (BF16 3x3 convolutions, 1 image 256^2)",
+        "roofline": {"bound": "mfma", "kernel": f"U-Net forward (BF16 3x3 convolutions, {ni} image(s) 256^2)",
                      "achieved": round(achieved, 3), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(achieved / 2500.0, 5), "traffic": None,
                      "flops_per_forward": flops},
@@ -217,10 +256,17 @@ def main() -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--rollout", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--macro", type=int, default=0,
+                    help="macro-step fields: 1 positions from the predicted displacement, 2 F_E from its gradient "
+                         "(either: the network also predicts the displacement, a second image per sample)")
+    ap.add_argument("--refine", type=int, default=0,
+                    help="Jacobi steps of the least-squares fit of the target fields (0: the transfers)")
+    ap.add_argument("--macro-variants", default="",
+                    help="comma-separated further macro flags to roll the held-out scenes out with")
     ap.add_argument("--tile", type=int, default=512)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device copy (rocprofv3 counter calibration)")
     args = ap.parse_args()

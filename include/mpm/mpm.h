@@ -184,11 +184,15 @@ int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* ini
  * Learned grid surrogate (no reference counterpart).  A 4-level BF16 U-Net
  * (32-64-128-128 channels, 3x3 convolutions, GELU; FP32 accumulation on the matrix
  * cores) over the grid nodes of a 2D quadratic-spline scene of height x width cells
- * predicts the grid velocity k substeps ahead from node features [mass, velocity,
- * mean J_P, mean log singular value of F_E].  Its training set is captured from
- * solvers into device-resident slots, training runs on the device (forward,
- * backward, Adam), and mpm_surrogate_advance moves a single-scene solver k
- * substeps with the predicted velocity or, when the residual test fails, with k
+ * predicts the grid velocity (and, optionally, the grid displacement: a second
+ * image of the batch, told apart by a constant mode plane) k substeps ahead from
+ * node features
+ * [mass, velocity, mean J_P, mean log singular value of F_E].  Its training set is
+ * captured from solvers into device-resident slots (targets: transfers, optionally
+ * least-squares fits, of the particles' velocities and displacements), training
+ * runs on the device
+ * (forward, backward, Adam), and mpm_surrogate_advance moves a single-scene solver
+ * k substeps with the predicted fields or, when the residual test fails, with k
  * plain substeps.
  */
 typedef struct mpm_surrogate mpm_surrogate;
@@ -196,7 +200,9 @@ typedef struct mpm_surrogate mpm_surrogate;
 enum { MPM_ADVANCE_PLAIN = 0, MPM_ADVANCE_NETWORK = 1, MPM_ADVANCE_BALLISTIC = 2 };
 
 /* height, width: image of grid nodes (multiples of 8; a scene must have WIDTH =
- * width and HEIGHT = height cells); max_batch: images per training step; capacity:
+ * width and HEIGHT = height cells); max_batch: slots per training step (one or two
+ * network images each; the host-data calls below take up to max_batch images and
+ * mpm_surrogate_time up to 2 max_batch); capacity:
  * training-set slots; seed: weight initialisation. */
 int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uint64_t seed,
                          mpm_surrogate** out);
@@ -222,17 +228,34 @@ int mpm_surrogate_forward(mpm_surrogate* g, int nb, const uint16_t* input, const
 int mpm_surrogate_gradient(mpm_surrogate* g, int nb, const uint16_t* input, const float* wm,
                            const float* target, float* loss, float* grad);
 /* Capture scene `scene` of solver s into training slot `slot`: what = 0 the node
- * features now (and the particle positions), what = 1 the target velocity now,
- * at the positions of the last what = 0 capture; k_substeps = the horizon. */
+ * features now (and the particle positions), what = 1 the target velocity and
+ * displacement now (since the last what = 0 capture), as grid fields at the
+ * positions of that capture (refined per mpm_surrogate_set_targets); k_substeps =
+ * the horizon. */
 int mpm_surrogate_record(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int what, int k_substeps);
-/* Raw sample of a slot: host [7][h*w] (mass, vx, vy, J_P, log sigma, target vx, vy)
+/* Raw sample of a slot: host [9][h*w] (mass, vx, vy, J_P, log sigma, target vx, vy,
+ * target displacement x, y)
  * and scalars [6] (particle mass, grid mass, wall-band mass, horizon T, gravity,
  * m_ref); either may be NULL.  slot == capacity reads the advance call's sample. */
 int mpm_surrogate_get_sample(mpm_surrogate* g, int slot, float* host, float* scalars);
 /* One Adam step (lr > 0) on the batch of slots, or the loss only (lr == 0). */
 int mpm_surrogate_train(mpm_surrogate* g, const int* slots, int nb, float lr, float* loss);
-/* Loss of the zero increment (ballistic grid velocity) on the same samples. */
+/* Loss of the zero increment (ballistic grid velocity and displacement) on the
+ * same samples. */
 int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, float* loss);
+/* Which predicted field the macro step of mpm_surrogate_advance takes each update
+ * from (no reference counterpart): MPM_MACRO_DISPLACEMENT_X moves the particles by
+ * the gathered displacement (else x' = x + T (v + v') / 2), MPM_MACRO_DISPLACEMENT_F
+ * advances F_E by the displacement gradient (else by T C').  v' and C' always come
+ * from the predicted velocity.  With either flag the network runs on two images per
+ * sample (velocity and displacement) in training and in the macro step; set it
+ * before training.  Default 0: velocity only, one image. */
+enum { MPM_MACRO_DISPLACEMENT_X = 1, MPM_MACRO_DISPLACEMENT_F = 2 };
+int mpm_surrogate_set_macro(mpm_surrogate* g, int flags);
+/* Jacobi steps (0..64) of the mass-weighted least-squares fit applied to the target
+ * fields of later what = 1 captures (no reference counterpart); 0 (default) keeps
+ * the transfers sum_p w_ip m_p y_p / m_i. */
+int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters);
 /* Advance a single-scene solver by k_substeps: mode MPM_ADVANCE_NETWORK or
  * MPM_ADVANCE_BALLISTIC (zero increment) when the residual (share of the mass in
  * the wall band or outside the image) is <= tol, else k plain substeps;

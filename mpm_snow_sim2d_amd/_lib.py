@@ -34,7 +34,7 @@ EXPORTS = [
     "mpm_check_finite", "mpm_sort_pairs_host", "mpm_last_error",
     "mpm_batch_create", "mpm_num_scenes", "mpm_scene_range", "mpm_set_graph", "mpm_autotune",
     "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
-    "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_set_conv", "mpm_surrogate_forward", "mpm_surrogate_gradient",
+    "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_set_conv", "mpm_surrogate_set_macro", "mpm_surrogate_set_targets", "mpm_surrogate_forward", "mpm_surrogate_gradient",
     "mpm_surrogate_record", "mpm_surrogate_get_sample", "mpm_surrogate_train", "mpm_surrogate_baseline_loss",
     "mpm_surrogate_advance", "mpm_surrogate_time",
     "mpm_save_snapshot", "mpm_snapshot_info", "mpm_load_snapshot",
@@ -119,6 +119,8 @@ def load() -> ctypes.CDLL:
         "mpm_surrogate_set_params": (i32, [vp, vp]),
         "mpm_surrogate_set_scale": (i32, [vp, ctypes.c_float]),
         "mpm_surrogate_set_conv": (i32, [vp, i32]),
+        "mpm_surrogate_set_macro": (i32, [vp, i32]),
+        "mpm_surrogate_set_targets": (i32, [vp, i32]),
         "mpm_surrogate_forward": (i32, [vp, i32, vp, vp, vp]),
         "mpm_surrogate_gradient": (i32, [vp, i32, vp, vp, vp, vp, vp]),
         "mpm_surrogate_record": (i32, [vp, vp, i32, i32, i32, i32]),

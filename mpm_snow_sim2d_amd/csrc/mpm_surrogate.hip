@@ -1,23 +1,37 @@
 // mpm_surrogate.hip — the learned grid surrogate of the 2D solver: grid features
 // captured from a solver's particles, a device-resident training set, training
 // steps of the U-Net (mpm_surrogate_net.hip), and the macro step that advances a
-// scene by k substeps with the predicted grid velocity, with the plain solver as
-// fallback.
+// scene by k substeps with the predicted grid velocity and displacement, with the
+// plain solver as fallback.
 //
 // Image: grid nodes (i, j) with 0 <= i < w, 0 <= j < h of a scene whose grid has
 // w = WIDTH cells (nodes 0..WIDTH); the last node row and column lie inside the
 // wall band and are left to the plain solver.
 //
-// Per-node raw features of a sample (FP32 [7][h*w]): mass m_i = sum_p w_ip m_p,
+// Per-node raw features of a sample (FP32 [9][h*w]): mass m_i = sum_p w_ip m_p,
 // velocity v_i = sum_p w_ip m_p v_p / m_i, mean J_P, mean log singular value of F_E
-// (= log|det F_E| / 2), and the Lagrangian target velocity
-//   t_i = sum_p w_ip(t) m_p v_p(t + k dt) / m_i
-// (weights of the positions at capture time, velocities k substeps later).
+// (= log|det F_E| / 2), and two Lagrangian target fields, velocity and
+// displacement k substeps later, y_p = v_p(t + k dt) and y_p = x_p(t + k dt) - x_p(t),
+// at the weights w_ip(t) of the positions at capture time.  Each target field is
+// the mass-weighted least-squares fit of the particle values,
+//   min_t sum_p m_p |y_p - sum_i w_ip t_i|^2,
+// approximated by refine_iters Jacobi-preconditioned Richardson steps from the
+// transfer t_i = sum_p w_ip m_p y_p / m_i (the preconditioner, the diagonal of the
+// normal equations' row sums, is m_i):
+//   t_i <- t_i + sum_p w_ip m_p (y_p - sum_j w_jp t_j) / m_i.
+// Gathered back to the particles, the fitted fields reproduce the particles'
+// values instead of smoothing them twice, and each step keeps sum_i m_i t_i
+// (refine_iters = 0, the default, keeps the transfers themselves).
 // Network input (BF16, 32 channels, normalised): [m/m_ref, v/v_ref, (J-1)*10,
-// logsig*10, occupancy, 0...]; output: the increment over ballistic motion,
-//   v_pred,i = v_i + g k dt e_y + d_ref * out'_i,
-// where out' is projected so that sum_i m_i out'_i = 0 (the prediction changes the
-// momentum of the scene by exactly the gravity impulse).
+// logsig*10, occupancy, mode, 0...]; the network predicts the velocity field and,
+// when the macro step uses the displacement (macro flags), also the displacement
+// field, selected by the constant mode plane (0 velocity, 1 displacement) of a
+// second image in the same batch, as the increment over ballistic motion,
+//   v_pred,i = v_i + g k dt e_y + d_ref * out'_i                        (mode 0)
+//   u_pred,i = k dt v_i + g dt^2 k (k + 1) / 2 e_y + d_ref k dt * out'_i (mode 1)
+// (the ballistic terms are the solver's own free-fall recurrence), where out' is
+// projected so that sum_i m_i out'_i = 0 (the prediction changes the momentum of
+// the scene by exactly the gravity impulse).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +52,9 @@ using unet::bf16;
 
 namespace {
 
-constexpr int NCH = 7;           // raw sample channels
+constexpr int NCH = 9;           // raw sample channels
+constexpr int MACRO_X = MPM_MACRO_DISPLACEMENT_X;   // macro step: positions from the displacement
+constexpr int MACRO_F = MPM_MACRO_DISPLACEMENT_F;   // macro step: F_E from the displacement gradient
 constexpr float V_REF = 3.0f;    // velocity scale of the input (m/s)
 
 struct SlotInfo {
@@ -48,7 +64,8 @@ struct SlotInfo {
     float T;               // k * dt
     float gravity;
     float m_ref;           // RHO * dx^2
-    float pad[2];
+    float ug;              // g dt^2 k (k + 1) / 2: free-fall displacement in k substeps
+    float pad;
 };
 
 int sfail(int code, const std::string& m) { return mpm::detail::set_error(code, m); }
@@ -125,7 +142,7 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
                                                 float band, float* __restrict__ sample, float* __restrict__ xs,
                                                 float2* __restrict__ part)
 {
-    constexpr int NCAP = MODE == 0 ? 5 : 2;   // channels scattered
+    constexpr int NCAP = MODE == 0 ? 5 : 4;   // channels scattered
     __shared__ float sh[4];
     __shared__ float s_bound[NCAP][4];
     __shared__ int s_box[4][4];
@@ -147,6 +164,7 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         const int p = first + k;
         const uint32_t id = P.pid[p];
         float x0 = P.x[p], x1 = P.x[n + p];
+        const float c0 = x0, c1 = x1;   // positions now
         if constexpr (MODE == 0) {
             xs[2 * (size_t)id] = x0;
             xs[2 * (size_t)id + 1] = x1;
@@ -170,6 +188,8 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
         } else {
             val[0] = m * v0;
             val[1] = m * v1;
+            val[2] = m * (c0 - x0);
+            val[3] = m * (c1 - x1);
         }
     }
     // the workgroup's largest |value| per channel (-> fixed-point scales) and the box
@@ -288,7 +308,7 @@ __global__ void __launch_bounds__(256) k_capture(DevParticles P, int first, int 
     }
 }
 
-// Momentum sums to per-node means (channels 1-4 in mode 0, 5-6 in mode 1), and
+// Momentum sums to per-node means (channels 1-4 in mode 0, 5-8 in mode 1), and
 // per-workgroup sums of the image's grid mass.
 __global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ sample, int hw, float* __restrict__ part,
                                                        int mode)
@@ -302,12 +322,70 @@ __global__ void __launch_bounds__(256) k_capture_finish(float* __restrict__ samp
         if (mode == 0) {
             for (int c = 1; c <= 4; ++c) sample[(size_t)c * hw + q] *= inv;
         } else {
-            for (int c = 5; c <= 6; ++c) sample[(size_t)c * hw + q] *= inv;
+            for (int c = 5; c <= 8; ++c) sample[(size_t)c * hw + q] *= inv;
         }
     }
     if (mode == 0) {
         const float a = block_sum(m, sh);
         if (threadIdx.x == 0) part[blockIdx.x] = a;
+    }
+}
+
+// One Richardson step of the target fields' least-squares fit, first half: per
+// particle of [first, first + count), the residual r_p = y_p - sum_i w_ip t_i of the
+// four target channels (velocity now, displacement since the saved position) at the
+// saved position, scattered as w_ip m_p r_p into acc [4][h*w].  Nodes outside the
+// image take no part, in the gather as in the scatter (as in the capture).
+__global__ void __launch_bounds__(256) k_jacobi_residual(DevParticles P, int first, int count, Phys ph, int h,
+                                                        int w, const float* __restrict__ sample,
+                                                        const float* __restrict__ xs, float* __restrict__ acc)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const int p = first + k, n = P.n;
+    const size_t hw = (size_t)h * w;
+    const uint32_t id = P.pid[p];
+    const float s0 = xs[2 * (size_t)id], s1 = xs[2 * (size_t)id + 1];
+    float r[4] = {P.v[p], P.v[n + p], P.x[p] - s0, P.x[n + p] - s1};
+    const float m = P.mass[p];
+    const float xg[2] = {s0 * ph.inv_dx, s1 * ph.inv_dx};
+    Stencil<2, 2> st;
+    st.init(xg);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int i = st.base[0] + a, j = st.base[1] + b;
+            if (i < 0 || i >= w || j < 0 || j >= h) continue;
+            const float wt = st.w[0][a] * st.w[1][b];
+            const size_t q = (size_t)j * w + i;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r[c] -= wt * sample[(5 + c) * hw + q];
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int i = st.base[0] + a, j = st.base[1] + b;
+            if (i < 0 || i >= w || j < 0 || j >= h) continue;
+            const float wm = st.w[0][a] * st.w[1][b] * m;
+            const size_t q = (size_t)j * w + i;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) atomicAdd(acc + c * hw + q, wm * r[c]);
+        }
+}
+
+// Second half: t_i += acc_i / m_i on occupied nodes; acc is left zero for the next step.
+__global__ void __launch_bounds__(256) k_jacobi_apply(float* __restrict__ sample, float* __restrict__ acc, int hw)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= hw) return;
+    const float m = sample[q];
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        sample[(size_t)(5 + c) * hw + q] += acc[(size_t)c * hw + q] * inv;
+        acc[(size_t)c * hw + q] = 0.f;
     }
 }
 
@@ -333,23 +411,26 @@ __global__ void __launch_bounds__(256) k_info_reduce(const float2* __restrict__ 
     }
 }
 
-// Batch of network inputs, projection weights and targets from dataset slots.
+// Batch of network inputs, projection weights and targets from dataset slots:
+// `images` (1 or 2) images per slot, image images * b + mode for slot b (mode 0
+// velocity, 1 displacement).
 __global__ void __launch_bounds__(256) k_pack(const float* __restrict__ data, const SlotInfo* __restrict__ info,
-                                             const int* __restrict__ slots, int nb, int hw, float d_ref,
+                                             const int* __restrict__ slots, int nb, int images, int hw, float d_ref,
                                              bf16* __restrict__ in, float* __restrict__ wm,
                                              float* __restrict__ target)
 {
-    const int64_t n = (int64_t)nb * hw;
+    const int64_t n = (int64_t)images * nb * hw;
     for (int64_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-        const int b = (int)(t / hw), q = (int)(t - (int64_t)b * hw);
-        const int slot = slots[b];
+        const int img = (int)(t / hw), q = (int)(t - (int64_t)img * hw);
+        const int mode = images == 2 ? (img & 1) : 0;
+        const int slot = slots[images == 2 ? img >> 1 : img];
         const float* s = data + (size_t)slot * NCH * hw;
         const SlotInfo I = info[slot];
         const float m = s[q];
         const bool occ = m > 0.f;
         float f[8] = {m / I.m_ref, s[hw + q] / V_REF, s[2 * hw + q] / V_REF,
                       occ ? (s[3 * (size_t)hw + q] - 1.f) * 10.f : 0.f, s[4 * (size_t)hw + q] * 10.f,
-                      occ ? 1.f : 0.f, 0.f, 0.f};
+                      occ ? 1.f : 0.f, (float)mode, 0.f};
         typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
         bf16x8 v0, z;
 #pragma unroll
@@ -364,60 +445,84 @@ __global__ void __launch_bounds__(256) k_pack(const float* __restrict__ data, co
         dst[3] = z;
         wm[t] = I.grid_mass > 0.f ? m / I.grid_mass : 0.f;
         if (target) {
-            target[2 * t] = occ ? (s[5 * (size_t)hw + q] - s[hw + q]) / d_ref : 0.f;
-            target[2 * t + 1] = occ ? (s[6 * (size_t)hw + q] - s[2 * (size_t)hw + q] - I.gravity * I.T) / d_ref : 0.f;
+            const float vx = s[hw + q], vy = s[2 * (size_t)hw + q];
+            if (mode == 0) {
+                target[2 * t] = occ ? (s[5 * (size_t)hw + q] - vx) / d_ref : 0.f;
+                target[2 * t + 1] = occ ? (s[6 * (size_t)hw + q] - vy - I.gravity * I.T) / d_ref : 0.f;
+            } else {
+                const float su = d_ref * I.T;
+                target[2 * t] = occ ? (s[7 * (size_t)hw + q] - I.T * vx) / su : 0.f;
+                target[2 * t + 1] = occ ? (s[8 * (size_t)hw + q] - I.T * vy - I.ug) / su : 0.f;
+            }
         }
     }
 }
 
 // Macro step: particles of [first, first+count) advanced by T = k dt with the
-// predicted grid velocity v_i + g T e_y + d_ref * out_i (out == NULL: ballistic).
+// predicted grid velocity v_pred,i = v_i + g T e_y + d_ref * outv_i and
+// displacement u_pred,i = T v_i + ug e_y + d_ref T * outu_i (outv == NULL: ballistic,
+// zero increments):
+//   x' = x + sum_i w_ip u_pred,i,   F_E' = clamp((I + D^-1 sum_i w_ip u_pred,i (x_i - x_p)^T) F_E),
+//   v' = sum_i w_ip v_pred,i,       C' = D^-1 sum_i w_ip v_pred,i (x_i - x_p)^T,
+// or, where flags leaves out MACRO_X / MACRO_F, the velocity-only forms
+// x' = x + T (v + v') / 2 and F_E' = clamp((I + T C') F_E);
+// with the singular values of F_E' clamped as in G2P (g2p.glsl:80-93) and J_P
+// scaled by the clamped ratio.  Nodes outside the image take the particle's own
+// ballistic motion.
 __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, int count, Phys ph, int h, int w,
-                                                  const float* __restrict__ sample, const float* __restrict__ out,
-                                                  float d_ref, float T)
+                                                  const float* __restrict__ sample, const float* __restrict__ outv,
+                                                  const float* __restrict__ outu, float d_ref, float T, float ug,
+                                                  int flags)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= count) return;
     const int p = first + k, n = P.n;
     const size_t hw = (size_t)h * w;
     const float xp[2] = {P.x[p], P.x[n + p]};
+    const float vp[2] = {P.v[p], P.v[n + p]};
     const float xg[2] = {xp[0] * ph.inv_dx, xp[1] * ph.inv_dx};
     Stencil<2, 2> st;
     st.init(xg);
-    float nv[2] = {0.f, 0.f}, nB[4] = {0.f, 0.f, 0.f, 0.f};
+    float nv[2] = {0.f, 0.f}, nB[4] = {0.f, 0.f, 0.f, 0.f}, nu[2] = {0.f, 0.f}, nG[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const int i = st.base[0] + a, j = st.base[1] + b;
             const float wgt = st.w[0][a] * st.w[1][b];
-            float vi[2] = {0.f, 0.f};
+            float vt[2] = {vp[0], vp[1]}, dv[2] = {0.f, 0.f}, du[2] = {0.f, 0.f};
             if (i >= 0 && i < w && j >= 0 && j < h) {
                 const size_t q = (size_t)j * w + i;
-                vi[0] = sample[hw + q];
-                vi[1] = sample[2 * hw + q];
-                if (out) {
-                    vi[0] += d_ref * out[2 * q];
-                    vi[1] += d_ref * out[2 * q + 1];
+                vt[0] = sample[hw + q];
+                vt[1] = sample[2 * hw + q];
+                if (outv) {
+                    dv[0] = d_ref * outv[2 * q];
+                    dv[1] = d_ref * outv[2 * q + 1];
                 }
-            } else {
-                vi[0] = P.v[p];
-                vi[1] = P.v[n + p];
+                if (outu) {
+                    du[0] = d_ref * T * outu[2 * q];
+                    du[1] = d_ref * T * outu[2 * q + 1];
+                }
             }
-            vi[1] += ph.gravity * T;
+            const float vi[2] = {vt[0] + dv[0], vt[1] + ph.gravity * T + dv[1]};
+            const float ui[2] = {T * vt[0] + du[0], T * vt[1] + ug + du[1]};
             const float dp[2] = {st.d[0][a] * ph.dx, st.d[1][b] * ph.dx};
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 nv[r] += wgt * vi[r];
+                nu[r] += wgt * ui[r];
 #pragma unroll
-                for (int c = 0; c < 2; ++c) nB[r * 2 + c] += wgt * (vi[r] * (-dp[c]));
+                for (int c = 0; c < 2; ++c) {
+                    nB[r * 2 + c] += wgt * (vi[r] * (-dp[c]));
+                    nG[r * 2 + c] += wgt * (ui[r] * (-dp[c]));
+                }
             }
         }
     float Cn[4], M[4], Fo[4], Fh[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         Cn[e] = ph.Dinv * nB[e];
-        M[e] = ((e % 3) == 0 ? 1.f : 0.f) + T * Cn[e];
+        M[e] = ((e % 3) == 0 ? 1.f : 0.f) + ((flags & MACRO_F) ? ph.Dinv * nG[e] : T * Cn[e]);
         Fo[e] = P.F[e * n + p];
     }
     matmul<2>(M, Fo, Fh);
@@ -426,10 +531,10 @@ __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, in
     float ratio = 1.f;
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
-        float s = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
-        s = fmaxf(s, 1e-6f);
-        sc[d] = s;
-        ratio *= sig[d] / s;
+        float sv = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
+        sv = fmaxf(sv, 1e-6f);
+        sc[d] = sv;
+        ratio *= sig[d] / sv;
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
@@ -438,7 +543,7 @@ __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, in
             P.F[(r * 2 + c) * n + p] = (U[r * 2] * sc[0]) * V[c * 2] + (U[r * 2 + 1] * sc[1]) * V[c * 2 + 1];
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
-        P.x[d * n + p] = xp[d] + 0.5f * T * (P.v[d * n + p] + nv[d]);
+        P.x[d * n + p] = (flags & MACRO_X) ? xp[d] + nu[d] : xp[d] + 0.5f * T * (vp[d] + nv[d]);
         P.v[d * n + p] = nv[d];
     }
 #pragma unroll
@@ -446,7 +551,7 @@ __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, in
     P.Jp[p] = P.Jp[p] * ratio;
 }
 
-__global__ void k_init_info(SlotInfo* info, float T, float g, float m_ref)
+__global__ void k_init_info(SlotInfo* info, float T, float g, float m_ref, float ug)
 {
     info->particle_mass = 0.f;
     info->grid_mass = 0.f;
@@ -454,6 +559,8 @@ __global__ void k_init_info(SlotInfo* info, float T, float g, float m_ref)
     info->T = T;
     info->gravity = g;
     info->m_ref = m_ref;
+    info->ug = ug;
+    info->pad = 0.f;
 }
 
 __global__ void __launch_bounds__(256) k_zero(float* p, int64_t n)
@@ -470,13 +577,17 @@ struct mpm_surrogate {
     UNet net;
     int h = 0, w = 0, max_batch = 0, capacity = 0;
     float d_ref = 0.1f;
+    int macro_flags = 0;           // MPM_MACRO_DISPLACEMENT_* of the macro step
+    int refine_iters = 0;          // least-squares refinement steps of the target fields
+    int images() const { return macro_flags ? 2 : 1; }   // network images per slot
     float* data = nullptr;         // (capacity + 1) slots x NCH x h*w; the last is the live slot
     SlotInfo* info = nullptr;      // capacity + 1
     float* xs = nullptr;           // saved positions by original particle index
     int64_t xs_cap = 0;
     float2* part_p = nullptr;      // per-workgroup particle / wall-band mass of a capture (xs_cap / 256 + 1)
+    float* acc = nullptr;          // [4][h*w] scatter of the target fit's residuals (zero between uses)
     float* part_g = nullptr;       // per-workgroup grid mass of a capture (h * w / 256 + 1)
-    bf16* in = nullptr;
+    bf16* in = nullptr;            // network batch: 2 images (velocity, displacement) per slot
     float* wm = nullptr;
     float* target = nullptr;
     int* slots_d = nullptr;
@@ -541,7 +652,9 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     const int first = (int)s->p_off[scene], count = (int)(s->p_off[scene + 1] - s->p_off[scene]);
     if (mode == 0) {
         k_zero<<<std::min((NCH * hw + 255) / 256, 4096), 256, 0, st>>>(sample, (int64_t)NCH * hw);
-        k_init_info<<<1, 1, 0, st>>>(info, (float)k_substeps * P.DELTA_TIME, P.GRAVITY, P.RHO * P.dx * P.dx);
+        const float dt = P.DELTA_TIME;
+        k_init_info<<<1, 1, 0, st>>>(info, (float)k_substeps * dt, P.GRAVITY, P.RHO * P.dx * P.dx,
+                                     P.GRAVITY * dt * dt * (0.5f * (float)k_substeps * (float)(k_substeps + 1)));
     }
     const float band = P.BORDER_MARGIN + 2.0f;
     const int nbp = (count + 255) / 256, nbg = (hw + 255) / 256;
@@ -555,6 +668,14 @@ int capture(mpm_surrogate* g, mpm_solver* s, int scene, int slot, int mode, int 
     }
     k_capture_finish<<<nbg, 256, 0, st>>>(sample, hw, g->part_g, mode);
     if (mode == 0) k_info_reduce<<<1, 256, 0, st>>>(g->part_p, nbp, g->part_g, nbg, info);
+    if (mode == 1 && count > 0) {
+        // least-squares refinement of the target fields (channels 5-8)
+        for (int it = 0; it < g->refine_iters; ++it) {
+            k_jacobi_residual<<<nbp, 256, 0, st>>>(s->dev_particles(), first, count, ph, g->h, g->w, sample, g->xs,
+                                                   g->acc);
+            k_jacobi_apply<<<nbg, 256, 0, st>>>(sample, g->acc, hw);
+        }
+    }
     S_TRY(hipGetLastError());
     if (!on_solver_stream) {
         S_TRY(hipEventRecord(g->ev_capture, g->st));
@@ -595,14 +716,15 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
     if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "hipStreamCreate failed"));
     std::string err;
-    int r = g->net.init(height, width, max_batch, seed, g->st, &err);
+    int r = g->net.init(height, width, 2 * max_batch, seed, g->st, &err);
     if (r) return bail(sfail(r == -1 ? MPM_E_INVALID : MPM_E_HIP, err));
     const size_t hw = (size_t)height * width;
     if (hipMalloc(&g->data, (size_t)(capacity + 1) * NCH * hw * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->info, (size_t)(capacity + 1) * sizeof(SlotInfo)) != hipSuccess ||
-        hipMalloc(&g->in, (size_t)max_batch * hw * 32 * sizeof(bf16)) != hipSuccess ||
-        hipMalloc(&g->wm, (size_t)max_batch * hw * sizeof(float)) != hipSuccess ||
-        hipMalloc(&g->target, (size_t)max_batch * hw * 2 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->in, (size_t)2 * max_batch * hw * 32 * sizeof(bf16)) != hipSuccess ||
+        hipMalloc(&g->wm, (size_t)2 * max_batch * hw * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->target, (size_t)2 * max_batch * hw * 2 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&g->acc, (size_t)4 * hw * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->slots_d, (size_t)max_batch * sizeof(int)) != hipSuccess ||
         hipMalloc(&g->loss_d, 256) != hipSuccess ||
         hipMalloc(&g->part_g, (hw / 256 + 1) * sizeof(float)) != hipSuccess ||
@@ -615,9 +737,10 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
     const int live_slot = capacity;
     if (hipMemcpyAsync(g->live_slot_d, &live_slot, sizeof(int), hipMemcpyHostToDevice, g->st) != hipSuccess ||
         hipMemsetAsync(g->info, 0, (size_t)(capacity + 1) * sizeof(SlotInfo), g->st) != hipSuccess ||
-        hipMemsetAsync(g->in, 0, (size_t)max_batch * hw * 32 * sizeof(bf16), g->st) != hipSuccess ||
-        hipMemsetAsync(g->wm, 0, (size_t)max_batch * hw * sizeof(float), g->st) != hipSuccess ||
-        hipMemsetAsync(g->target, 0, (size_t)max_batch * hw * 2 * sizeof(float), g->st) != hipSuccess ||
+        hipMemsetAsync(g->in, 0, (size_t)2 * max_batch * hw * 32 * sizeof(bf16), g->st) != hipSuccess ||
+        hipMemsetAsync(g->wm, 0, (size_t)2 * max_batch * hw * sizeof(float), g->st) != hipSuccess ||
+        hipMemsetAsync(g->target, 0, (size_t)2 * max_batch * hw * 2 * sizeof(float), g->st) != hipSuccess ||
+        hipMemsetAsync(g->acc, 0, (size_t)4 * hw * sizeof(float), g->st) != hipSuccess ||
         hipStreamSynchronize(g->st) != hipSuccess)
         return bail(sfail(MPM_E_HIP, "surrogate initialisation failed"));
     *out = g;
@@ -637,6 +760,7 @@ int mpm_surrogate_destroy(mpm_surrogate* g)
     (void)hipFree(g->in);
     (void)hipFree(g->wm);
     (void)hipFree(g->target);
+    (void)hipFree(g->acc);
     (void)hipFree(g->slots_d);
     (void)hipFree(g->live_slot_d);
     (void)hipFree(g->loss_d);
@@ -670,6 +794,22 @@ int mpm_surrogate_set_params(mpm_surrogate* g, const float* host)
     S_TRY(hipMemcpyAsync(g->net.params(), host, g->net.num_params() * 4, hipMemcpyHostToDevice, g->st));
     S_TRY(g->net.sync_weights());
     S_TRY(hipStreamSynchronize(g->st));
+    return MPM_OK;
+}
+
+int mpm_surrogate_set_macro(mpm_surrogate* g, int flags)
+{
+    if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    if (flags & ~(MPM_MACRO_DISPLACEMENT_X | MPM_MACRO_DISPLACEMENT_F)) return sfail(MPM_E_INVALID, "unknown macro flags");
+    g->macro_flags = flags;
+    return MPM_OK;
+}
+
+int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters)
+{
+    if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    if (refine_iters < 0 || refine_iters > 64) return sfail(MPM_E_INVALID, "refine_iters must be in [0, 64]");
+    g->refine_iters = refine_iters;
     return MPM_OK;
 }
 
@@ -754,11 +894,12 @@ int mpm_surrogate_train(mpm_surrogate* g, const int* slots, int nb, float lr, fl
     int r = upload_slots(g, slots, nb);
     if (r) return r;
     const int hw = g->h * g->w;
-    k_pack<<<std::min((nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, hw, g->d_ref,
-                                                                     g->in, g->wm, g->target);
+    const int ni = g->images();
+    k_pack<<<std::min((ni * nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, ni, hw,
+                                                                          g->d_ref, g->in, g->wm, g->target);
     S_TRY(hipGetLastError());
     S_TRY(unet::fill_f32(g->loss_d, 1, 0.f, g->st));
-    S_TRY(g->net.loss_and_backward(nb, g->in, g->wm, g->target, g->loss_d, lr > 0.f));
+    S_TRY(g->net.loss_and_backward(ni * nb, g->in, g->wm, g->target, g->loss_d, lr > 0.f));
     if (lr > 0.f) S_TRY(g->net.adam(lr));
     if (loss) {
         S_TRY(hipMemcpyAsync(loss, g->loss_d, 4, hipMemcpyDeviceToHost, g->st));
@@ -774,23 +915,24 @@ int mpm_surrogate_baseline_loss(mpm_surrogate* g, const int* slots, int nb, floa
     int r = upload_slots(g, slots, nb);
     if (r) return r;
     const int hw = g->h * g->w;
-    k_pack<<<std::min((nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, hw, g->d_ref,
-                                                                     g->in, g->wm, g->target);
+    const int ni = g->images();
+    k_pack<<<std::min((ni * nb * hw + 255) / 256, 8192), 256, 0, g->st>>>(g->data, g->info, g->slots_d, nb, ni, hw,
+                                                                          g->d_ref, g->in, g->wm, g->target);
     S_TRY(hipGetLastError());
-    std::vector<float> t((size_t)nb * hw * 2), wm((size_t)nb * hw);
+    std::vector<float> t((size_t)ni * nb * hw * 2), wm((size_t)ni * nb * hw);
     S_TRY(hipMemcpyAsync(t.data(), g->target, t.size() * 4, hipMemcpyDeviceToHost, g->st));
     S_TRY(hipMemcpyAsync(wm.data(), g->wm, wm.size() * 4, hipMemcpyDeviceToHost, g->st));
     S_TRY(hipStreamSynchronize(g->st));
     double l = 0.0;
     for (size_t i = 0; i < wm.size(); ++i) l += (double)wm[i] * ((double)t[2 * i] * t[2 * i] + (double)t[2 * i + 1] * t[2 * i + 1]);
-    *loss = (float)(l / nb);
+    *loss = (float)(l / (ni * nb));
     return MPM_OK;
 }
 
 int mpm_surrogate_time(mpm_surrogate* g, int nb, int iters, int train, double* ms)
 {
     if (!g || !ms) return sfail(MPM_E_INVALID, "NULL argument");
-    if (nb < 1 || nb > g->max_batch || iters < 1) return sfail(MPM_E_INVALID, "nb/iters out of range");
+    if (nb < 1 || nb > g->net.max_batch() || iters < 1) return sfail(MPM_E_INVALID, "nb/iters out of range");
     hipEvent_t a = nullptr, b = nullptr;
     S_TRY(hipEventCreate(&a));
     S_TRY(hipEventCreate(&b));
@@ -837,10 +979,12 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     // only the live slot; if the test fails its output is not used)
     const float* out = nullptr;
     if (mode == MPM_ADVANCE_NETWORK) {
-        k_pack<<<std::min((hw + 255) / 256, 8192), 256, 0, st>>>(g->data, g->info, g->live_slot_d, 1, hw, g->d_ref,
-                                                                 g->in, g->wm, nullptr);
+        // one batch: velocity (image 0) and, when used, displacement (image 1)
+        const int ni = g->images();
+        k_pack<<<std::min((ni * hw + 255) / 256, 8192), 256, 0, st>>>(g->data, g->info, g->live_slot_d, 1, ni, hw,
+                                                                      g->d_ref, g->in, g->wm, nullptr);
         S_TRY(hipGetLastError());
-        S_TRY(g->net.forward(1, g->in, g->wm, st));
+        S_TRY(g->net.forward(ni, g->in, g->wm, st));
         out = g->net.out();
     }
     S_TRY(hipEventSynchronize(g->ev_info));
@@ -859,9 +1003,12 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
 
     const Phys ph = s->scenes_h[0].ph;
     const float T = (float)k_substeps * ph.dt;
+    const float ug = ph.gravity * ph.dt * ph.dt * (0.5f * (float)k_substeps * (float)(k_substeps + 1));
     const float* sample = g->data + (size_t)g->live() * NCH * hw;
     k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
-                                                    g->d_ref, T);
+                                                    out && g->images() == 2 ? out + (size_t)2 * hw : nullptr, g->d_ref,
+                                                    T, ug,
+                                                    g->macro_flags);
     S_TRY(hipGetLastError());
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;

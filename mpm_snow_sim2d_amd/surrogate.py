@@ -1,5 +1,8 @@
 """Learned grid surrogate over the C-ABI (``mpm_surrogate_*``): a BF16 U-Net on the
-matrix cores that predicts the grid velocity k substeps ahead, trained on the
+matrix cores that predicts the grid velocity and displacement k substeps ahead
+(transfers or least-squares fits of the particles' values; the displacement,
+when the macro step uses it, as a second image of the batch told apart by a
+constant mode plane), trained on the
 device from trajectories the solver generates, and a hybrid rollout that uses it
 where its residual test passes and the plain solver elsewhere.
 
@@ -22,6 +25,7 @@ ADVANCE_PLAIN = _lib.ADVANCE_PLAIN
 ADVANCE_NETWORK = _lib.ADVANCE_NETWORK
 ADVANCE_BALLISTIC = _lib.ADVANCE_BALLISTIC
 CONV_AUTO, CONV_STREAM, CONV_LDS = _lib.CONV_AUTO, _lib.CONV_STREAM, _lib.CONV_LDS
+MACRO_X, MACRO_F = 1, 2   # MPM_MACRO_DISPLACEMENT_X, MPM_MACRO_DISPLACEMENT_F
 
 
 def _ptr(a):
@@ -80,6 +84,17 @@ class Surrogate:
         self.d_ref = float(d_ref)
         check(self._lib.mpm_surrogate_set_scale(self._h, ctypes.c_float(d_ref)))
 
+    def set_macro(self, flags: int) -> None:
+        """Fields of the macro step: MACRO_X moves particles by the predicted
+        displacement (else by the trapezoid of the velocities), MACRO_F advances F_E by
+        its gradient (else by T C'); velocity and C always from the predicted velocity."""
+        check(self._lib.mpm_surrogate_set_macro(self._h, int(flags)))
+
+    def set_targets(self, refine_iters: int) -> None:
+        """Jacobi steps of the least-squares fit applied to later target captures
+        (0: the plain transfers)."""
+        check(self._lib.mpm_surrogate_set_targets(self._h, int(refine_iters)))
+
     def set_conv(self, kernel: int) -> None:
         """Forward convolution kernel: CONV_AUTO (by layer shape), CONV_STREAM
         (register-streaming) or CONV_LDS (LDS-tiled)."""
@@ -109,7 +124,7 @@ class Surrogate:
         check(self._lib.mpm_surrogate_record(self._h, solver._h, int(scene), int(slot), int(what), int(k)))
 
     def get_sample(self, slot: int):
-        data = np.empty((7, self.h * self.w), np.float32)
+        data = np.empty((9, self.h * self.w), np.float32)
         sc = np.empty(6, np.float32)
         check(self._lib.mpm_surrogate_get_sample(self._h, int(slot), _ptr(data), _ptr(sc)))
         keys = ("particle_mass", "grid_mass", "band_mass", "T", "gravity", "m_ref")
@@ -131,7 +146,8 @@ class Surrogate:
         return float(loss[0])
 
     def time(self, nb: int, iters: int = 10, train: bool = False) -> float:
-        """Average device ms of a forward (or forward + backward) pass on nb images."""
+        """Average device ms of a forward (or forward + backward) pass on nb images
+        (a macro step runs 1, or 2 with displacement; a training step as many per slot)."""
         ms = ctypes.c_double(0.0)
         check(self._lib.mpm_surrogate_time(self._h, int(nb), int(iters), int(train), ctypes.byref(ms)))
         return float(ms.value)

@@ -7,10 +7,17 @@
   significant bits; activations and activation gradients are rounded to it after
   every layer); the head bias, on which the projected loss does not depend, has a
   gradient of zero up to rounding.
-* Capture: node features and Lagrangian targets against a NumPy P2G of the same
-  particles (FP32 atomics against FP64 sums: relative 1e-5).
+* Capture: node features against a NumPy P2G of the same particles (FP32 atomics
+  against FP64 sums: relative 1e-5); the velocity and displacement targets, as
+  transfers and after 8 steps of their least-squares refinement, against a NumPy
+  restatement (relative 2e-4 of the field's largest value), which also shows the
+  refinement lowers the particles' velocity fit residual and keeps the fields'
+  mass-weighted means.
 * Macro step: ballistic advance of a stress-free body in free fall against its
-  closed form, fallback to plain substeps at the walls.
+  closed form (trapezoid positions, or the solver's own free-fall recurrence when
+  the displacement moves the particles), fallback to plain substeps at the walls.
+* Training: Adam fits a fixed batch with one image per sample and with the
+  velocity + displacement pair.
 * Ablation: a network trained on solver trajectories predicts the velocity of
   held-out colliding scenes better than the method without it (the ballistic grid
   velocity, i.e. a zero network output), and its hybrid rollout lands closer to
@@ -149,12 +156,17 @@ def test_network_gradient_matches_oracle():
         assert err < 2e-2 and cos >= 0.9995, k
 
 
-def test_training_steps_reduce_loss():
-    """Adam on the device fits a fixed batch of captured samples."""
+@pytest.mark.parametrize("flags,refine", [(0, 0), (3, 8)])
+def test_training_steps_reduce_loss(flags, refine):
+    """Adam on the device fits a fixed batch of captured samples (velocity only, and
+    velocity + displacement with refined targets)."""
     from mpm_snow_sim2d_amd.surrogate import Surrogate, generate
     P = snow_params(2, 32, 4e-4)
     scenes = [(P, colliding_scene(50 + i, 1500)) for i in range(2)]
+    _flush(f"[surrogate] fixed-batch training flags={flags} refine={refine}")
     with Surrogate(32, 32, max_batch=4, capacity=24, seed=3) as sur:
+        sur.set_macro(flags)
+        sur.set_targets(refine)
         # the last samples of 96 substeps, after the bodies have met
         slots = [ss[-2:] for ss in generate(sur, scenes, 96, 8)]
         slots = [s for ss in slots for s in ss]
@@ -188,14 +200,47 @@ def _p2g_numpy(P, x, vals, m, h, w):
     return acc
 
 
-def test_capture_matches_numpy():
+def _gather_numpy(P, x, field, h, w):
+    """sum_i w_ip field_i over nodes [0,w) x [0,h) (quadratic B-spline, FP64)."""
+    inv = 1.0 / P.dx
+    xg = x.astype(np.float64) * inv
+    base = np.floor(xg - 0.5).astype(np.int64)
+    out = np.zeros((field.shape[0], x.shape[1]))
+    for a in range(3):
+        for b in range(3):
+            i, j = base[0] + a, base[1] + b
+            di, dj = np.abs(xg[0] - i), np.abs(xg[1] - j)
+            wx = np.where(di < 0.5, 0.75 - di * di, np.where(di < 1.5, 0.5 * (1.5 - di) ** 2, 0.0))
+            wy = np.where(dj < 0.5, 0.75 - dj * dj, np.where(dj < 1.5, 0.5 * (1.5 - dj) ** 2, 0.0))
+            ok = (i >= 0) & (i < w) & (j >= 0) & (j < h)
+            q = np.where(ok, j * w + i, 0)
+            out += np.where(ok, wx * wy, 0.0) * field[:, q]
+    return out
+
+
+def _fit_numpy(P, x, y, m, h, w, iters=8):
+    """Mass-weighted least-squares grid fields of the particle values y (c, n): the
+    P2G transfer, then `iters` steps t += P2G(m (y - G2P(t))) / m_i."""
+    acc = _p2g_numpy(P, x, y, m, h, w)
+    mass = acc[0]
+    inv = np.where(mass > 0, 1.0 / np.where(mass > 0, mass, 1.0), 0.0)
+    t = acc[1:] * inv
+    for _ in range(iters):
+        r = y - _gather_numpy(P, x, t, h, w)
+        t = t + _p2g_numpy(P, x, r, m, h, w)[1:] * inv
+    return t
+
+
+@pytest.mark.parametrize("refine", [0, 8])
+def test_capture_matches_numpy(refine):
     from mpm_snow_sim2d_amd.solver import GPUSolver
     from mpm_snow_sim2d_amd.surrogate import Surrogate
     P = snow_params(2, 64, 2e-4)
     ps = colliding_scene(5, 3000)
     k = 8
-    _flush(f"[surrogate] capture n={ps.n}")
+    _flush(f"[surrogate] capture n={ps.n} refine={refine}")
     with Surrogate(64, 64, max_batch=1, capacity=2) as sur:
+        sur.set_targets(refine)
         s = GPUSolver(P, ps, Options(sort_every=4, tile_particles=256))
         s.Step(6)
         st0 = s.get_state()
@@ -209,7 +254,8 @@ def test_capture_matches_numpy():
     F = st0["F"].astype(np.float64)
     ls = 0.5 * np.log(np.abs(F[0] * F[3] - F[1] * F[2]))
     acc = _p2g_numpy(P, st0["x"], np.stack([st0["v"][0], st0["v"][1], st0["Jp"], ls]), m, 64, 64)
-    tgt = _p2g_numpy(P, st0["x"], st1["v"].astype(np.float64), m, 64, 64)
+    y = np.concatenate([st1["v"].astype(np.float64), st1["x"].astype(np.float64) - st0["x"]])
+    tgt = _fit_numpy(P, st0["x"], y, m, 64, 64, refine)
     mass = acc[0]
     occ = mass > 1e-6 * mass.max()
     assert np.allclose(data[0], mass, rtol=1e-5, atol=1e-6 * mass.max())
@@ -219,17 +265,29 @@ def test_capture_matches_numpy():
         # det F_E (6e-8) bounds it absolutely
         tol = max(1e-4 * np.abs(ref).max(), 1e-6)
         assert np.abs(data[c + 1][occ] - ref).max() <= tol, c
-    for c in range(2):
-        ref = tgt[c + 1][occ] / mass[occ]
-        assert np.abs(data[5 + c][occ] - ref).max() <= 1e-4 * np.abs(ref).max(), c
+    for c in range(4):
+        ref = tgt[c][occ]
+        assert np.abs(data[5 + c][occ] - ref).max() <= 2e-4 * np.abs(ref).max(), c
+    # the refinement fits the particles' velocities better than the plain transfer
+    # (the displacements of these still nearly rigid bodies sit at the rounding
+    # floor either way) and keeps each field's mass-weighted sum (all stencils lie
+    # inside the image here)
+    pic = _p2g_numpy(P, st0["x"], y, m, 64, 64)[1:] / np.where(mass > 0, mass, 1.0)
+    res_fit = np.sum(m * (y[:2] - _gather_numpy(P, st0["x"], tgt[:2], 64, 64)) ** 2)
+    res_pic = np.sum(m * (y[:2] - _gather_numpy(P, st0["x"], pic[:2], 64, 64)) ** 2)
+    if refine:
+        assert res_fit < 0.5 * res_pic
+    assert np.allclose((mass * tgt).sum(1), (m * y).sum(1), rtol=1e-9, atol=1e-12)
     assert abs(sc["particle_mass"] - m.sum()) <= 1e-5 * m.sum()
     assert abs(sc["T"] - k * np.float32(P.DELTA_TIME)) < 1e-9
     assert sc["band_mass"] == 0.0
 
 
-def test_ballistic_macro_step_free_fall():
-    """A stress-free body with uniform velocity: v' = v + g T, x' = x + T v + g T^2/2,
-    F and J_P unchanged."""
+@pytest.mark.parametrize("flags", [0, 3])
+def test_ballistic_macro_step_free_fall(flags):
+    """A stress-free body with uniform velocity: v' = v + g T; x' = x + T v + g T^2 / 2
+    (trapezoid), or, moved by the displacement, the solver's own free-fall
+    recurrence x' = x + T v + g dt^2 k (k + 1) / 2; F and J_P unchanged."""
     from mpm_snow_sim2d_amd.solver import GPUSolver
     from mpm_snow_sim2d_amd.surrogate import ADVANCE_BALLISTIC, Surrogate
     P = snow_params(2, 64, 2e-4)
@@ -239,6 +297,7 @@ def test_ballistic_macro_step_free_fall():
     ps = _set(pts, (0.7, -0.3), 0.004 ** 2, 400 * 0.004 ** 2)
     k = 8
     with Surrogate(64, 64, max_batch=1, capacity=0) as sur:
+        sur.set_macro(flags)
         s = GPUSolver(P, ps, Options(sort_every=4))
         st0 = s.get_state()
         used, res = sur.advance(s, k, ADVANCE_BALLISTIC, 1e-3)
@@ -249,7 +308,9 @@ def test_ballistic_macro_step_free_fall():
     g = np.array([0.0, P.GRAVITY])
     v0 = st0["v"].astype(np.float64)
     assert np.abs(st1["v"] - (v0 + g[:, None] * T)).max() < 1e-5
-    xr = st0["x"] + T * v0 + 0.5 * g[:, None] * T * T
+    dt = np.float32(P.DELTA_TIME)
+    fall = dt * dt * (k * (k + 1) / 2) if flags else 0.5 * T * T
+    xr = st0["x"] + T * v0 + g[:, None] * fall
     assert np.abs(st1["x"] - xr).max() < 1e-6
     assert np.abs(st1["F"] - st0["F"]).max() < 1e-5
     assert np.abs(st1["Jp"] - st0["Jp"]).max() < 1e-6
