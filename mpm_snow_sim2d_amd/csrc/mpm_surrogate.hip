@@ -813,6 +813,7 @@ int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters)
     return MPM_OK;
 }
 
+
 int mpm_surrogate_set_conv(mpm_surrogate* g, int kernel)
 {
     if (!g) return sfail(MPM_E_INVALID, "NULL argument");
@@ -995,21 +996,26 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     const float M = I.particle_mass > 0.f ? I.particle_mass : 1.f;
     const float res = std::max(I.band_mass / M, std::fabs(M - I.grid_mass) / M);
     if (residual) *residual = res;
-    // later work on the surrogate's stream waits for this macro step's use of the
-    // shared buffers
-    S_TRY(hipEventRecord(g->ev_capture, st));
-    S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
-    if (!(res <= tol)) return mpm_step(s, k_substeps);
+    if (!(res <= tol)) {
+        // later work on the surrogate's stream waits for this call's use of the
+        // shared buffers (capture, network)
+        S_TRY(hipEventRecord(g->ev_capture, st));
+        S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
+        return mpm_step(s, k_substeps);
+    }
 
     const Phys ph = s->scenes_h[0].ph;
     const float T = (float)k_substeps * ph.dt;
     const float ug = ph.gravity * ph.dt * ph.dt * (0.5f * (float)k_substeps * (float)(k_substeps + 1));
     const float* sample = g->data + (size_t)g->live() * NCH * hw;
-    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out,
-                                                    out && g->images() == 2 ? out + (size_t)2 * hw : nullptr, g->d_ref,
-                                                    T, ug,
-                                                    g->macro_flags);
+    const float* outu = out && g->images() == 2 ? out + (size_t)2 * hw : nullptr;
+    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out, outu,
+                                                    g->d_ref, T, ug, g->macro_flags);
     S_TRY(hipGetLastError());
+    // later work on the surrogate's stream (training, recording) waits for this
+    // macro step's use of the shared buffers: the live slot and the network output
+    S_TRY(hipEventRecord(g->ev_capture, st));
+    S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;
     s->items_valid = false;
