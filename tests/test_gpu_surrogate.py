@@ -316,6 +316,61 @@ def test_ballistic_macro_step_free_fall(flags):
     assert np.abs(st1["Jp"] - st0["Jp"]).max() < 1e-6
 
 
+@pytest.mark.parametrize("flags", [0, 3])
+def test_macro_step_matches_restatement(flags):
+    """The ballistic macro step of a deforming scene (two bodies meeting: non-uniform
+    velocity, C and F_E) against the NumPy restatement of k_macro_g2p
+    (tests/surrogate_floor.py) fed the transfer of the same state in FP64: x, v, C,
+    F_E and J_P, both forms (trapezoid / displacement)."""
+    import surrogate_floor as SF
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.surrogate import ADVANCE_BALLISTIC, Surrogate
+    from oracle import oracle as orc
+    P = snow_params(2, 64, 2e-4)
+    ps = colliding_scene(5, 3000)
+    k = 8
+    _flush(f"[surrogate] macro step vs restatement flags={flags}")
+    with Surrogate(64, 64, max_batch=1, capacity=0) as sur:
+        sur.set_macro(flags)
+        s = GPUSolver(P, ps, Options(sort_every=4, tile_particles=256))
+        s.Step(128)                 # the bodies have met and deform
+        st0 = s.get_state()
+        used, res = sur.advance(s, k, ADVANCE_BALLISTIC, 1e-3)
+        st1 = s.get_state()
+        s.close()
+    assert used == ADVANCE_BALLISTIC
+    n = ps.n
+    dinv = 4.0 / float(P.dx) ** 2
+    Fp = np.zeros((4, n))
+    Fp[0] = st0["Jp"]
+    Fp[3] = 1.0
+    ost = orc.OracleState(st0["x"].astype(np.float64), st0["v"].astype(np.float64),
+                          st0["C"].astype(np.float64) / dinv, st0["F"].astype(np.float64), Fp,
+                          ps.mass.astype(np.float64), ps.vol.astype(np.float64))
+    img = SF.Image(P, 64, 64)
+    sten = img.stencil(ost.x)
+    m = ost.mass
+    dt = float(np.float32(P.DELTA_TIME))
+    T, g = k * dt, float(np.float32(P.GRAVITY))
+    vf, _, _ = img.transfer(sten, m, ost.v)
+    vf[1] += g * T
+    uf = None
+    if flags:
+        uf, _, _ = img.transfer(sten, m, T * ost.v)
+        uf[1] += g * dt * dt * k * (k + 1) / 2
+    SF.macro_step(P, img, ost, k, vf, uf)
+    vscale = np.abs(ost.v).max()
+    assert np.abs(st1["x"] - ost.x).max() < 1e-6
+    assert np.abs(st1["v"] - ost.v).max() < 1e-4 * vscale
+    C = dinv * ost.B
+    assert np.abs(st1["C"] - C).max() < 1e-3 * max(np.abs(C).max(), 1.0)
+    assert np.abs(st1["F"] - ost.Fe).max() < 1e-5
+    Jp = ost.Fp[0] * ost.Fp[3] - ost.Fp[1] * ost.Fp[2]
+    assert np.abs(st1["Jp"] - Jp).max() < 1e-5
+    # the step is not trivial: the bodies deform
+    assert np.abs(st1["F"] - st0["F"]).max() > 1e-4
+
+
 def test_advance_falls_back_at_walls():
     from mpm_snow_sim2d_amd.solver import GPUSolver
     from mpm_snow_sim2d_amd.surrogate import ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate
