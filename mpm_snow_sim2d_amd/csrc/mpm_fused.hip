@@ -168,7 +168,6 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             dst = scene_grid(dst_all, sd);
             blk -= sd.block_off;
         }
-        const float fdt = ph.Dinv * ph.dt;
         int first = item.y, cnt = min(item.z, 256 * RMAX);
         int bc[3];
         bc[0] = blk % dst.nbx;
@@ -180,7 +179,12 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         for (int a = 0; a < 3; ++a) o[a] = (a < DIM) ? bc[a] * BS - 1 - MG : 0;
 
         // ---- stage the tile: velocities to gather from, zeroed accumulators (all
-        // loads issued before the first LDS store, so their latencies overlap)
+        // loads issued before the first LDS store, so their latencies overlap).
+        // The lane's tile entries are derived from an opaque copy of its index, so
+        // they are recomputed per item (a few integer operations) instead of being
+        // hoisted out of the item loop and kept live across the G2P phase.
+        int ltid = tid;
+        asm volatile("" : "+v"(ltid));
         {
             constexpr int NR = (TN + 255) / 256;
             float4 va[NR], vb[NR];
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             for (int k = 0; k < NR; ++k) {
                 va[k] = make_float4(0.f, 0.f, 0.f, 0.f);
                 vb[k] = va[k];
-                const int e = tid + 256 * k;
+                const int e = ltid + 256 * k;
                 if (G2P && e < TN) {
                     int t[3];
                     tile_coords<DIM>(e, t);
@@ -202,7 +206,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             }
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int e = tid + 256 * k;
+                const int e = ltid + 256 * k;
                 if (e >= TN) continue;
 #pragma unroll
                 for (int c = 0; c <= DIM; ++c) s_acc[c][e] = 0;
@@ -483,7 +487,11 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 
             if (vol == 0.f) continue;                                 // p2g.glsl:42
             // momentum of node i: w (m v + A (x_i - x_p)), A = m C - (V D^-1 dt) tau
-            const float vfd = vol * fdt;
+            // D^-1 dt formed here from the uniform parameters (an opaque copy keeps
+            // it from being hoisted out of the item loop into a long-lived register)
+            float dinv = ph.Dinv;
+            asm volatile("" : "+v"(dinv));
+            const float vfd = vol * (dinv * ph.dt);
             float bp = 0.f;
 #pragma unroll
             for (int rr = 0; rr < DIM; ++rr) {
@@ -666,8 +674,10 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             s_touch[(rbz * 3 + rby) * 3 + rbx] = 1;
         }
         lds_barrier();
-        if (tid < TS::NB && s_touch[tid]) {
-            int rbx = tid % 3, rby = (tid / 3) % 3, rbz = tid / 9;
+        int ftid = tid;                       // recomputed per item, as ltid above
+        asm volatile("" : "+v"(ftid));
+        if (ftid < TS::NB && s_touch[ftid]) {
+            int rbx = ftid % 3, rby = (ftid / 3) % 3, rbz = ftid / 9;
             int bx = bc[0] - 1 + rbx, by = bc[1] - 1 + rby, bz = DIM == 3 ? bc[2] - 1 + rbz : 0;
             int b = DIM == 3 ? (bz * dst.nby + by) * dst.nbx + bx : by * dst.nbx + bx;
             dst.flags[b] = 1u;
