@@ -371,6 +371,23 @@ def test_macro_step_matches_restatement(flags):
     assert np.abs(st1["F"] - st0["F"]).max() > 1e-4
 
 
+def test_option_setters_reject_bad_values():
+    """mpm_surrogate_set_macro / set_targets refuse values outside their range and
+    leave the surrogate usable."""
+    from mpm_snow_sim2d_amd.surrogate import MACRO_F, MACRO_X, Surrogate
+    with Surrogate(32, 32, max_batch=1, capacity=1) as sur:
+        for bad in (4, -1, 8):
+            with pytest.raises(RuntimeError):
+                sur.set_macro(bad)
+        for bad in (-1, 65):
+            with pytest.raises(RuntimeError):
+                sur.set_targets(bad)
+        sur.set_macro(MACRO_X | MACRO_F)
+        sur.set_targets(64)
+        sur.set_macro(0)
+        sur.set_targets(0)
+
+
 def test_advance_falls_back_at_walls():
     from mpm_snow_sim2d_amd.solver import GPUSolver
     from mpm_snow_sim2d_amd.surrogate import ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate
