@@ -64,3 +64,38 @@ def test_falling_cube_free_fall_closed_form():
     # a few hundred FP32 roundings of positions near 0.5 m, so x is bounded above
     # against the oracle only)
     assert dev["v"] < 0.05 * abs(v_exact[1, 0])
+
+
+@pytest.mark.timeout(600)
+def test_batch64_scenes_match_single_scene_runs():
+    """The benchmark's batched workload at full size (64 scenes, 1.70M particles,
+    128^2 / 256^2 grids, hipGraph replay): every scene of the batch against the same
+    scene run alone (eager), after 64 substeps, under the batch tests' bounds (max
+    1e-3, medians 1e-4 cells / 1e-5 in natural units: only the order of float
+    additions differs)."""
+    from mpm_snow_sim2d_amd.scenes import batch_scenes
+    from mpm_snow_sim2d_amd.solver import BatchSolver, GPUSolver
+    from tests.helpers import state_errors
+    scenes = batch_scenes(64, seed=0)
+    nsub = 64
+    print(f"[fullsize] batch of {len(scenes)} scenes, {sum(ps.n for _, ps in scenes)} particles, "
+          f"{nsub} substeps", flush=True)
+    opts = Options(sort_every=32, tile_particles=512)
+    with BatchSolver(scenes, opts, graph=True) as b:
+        b.Step(nsub)
+        st = b.get_state()
+    bound_med = {"x": 1e-4, "v": 1e-5, "C": 1e-5, "F": 1e-5, "Jp": 1e-5}
+    worst = {k: 0.0 for k in bound_med}
+    a = 0
+    for k, (P, ps) in enumerate(scenes):
+        c = ps.n
+        with GPUSolver(P, ps, opts) as s:
+            s.Step(nsub)
+            one = s.get_state()
+        err = state_errors({key: v[..., a:a + c] for key, v in st.items()}, one, P.dx)
+        a += c
+        for key in bound_med:
+            worst[key] = max(worst[key], err[key][0])
+            assert err[key][0] < 1e-3, (k, key, err[key])
+            assert err[key][1] < bound_med[key], (k, key, err[key])
+    print(f"    worst max difference per field: {worst}", flush=True)
