@@ -99,3 +99,38 @@ def test_batch64_scenes_match_single_scene_runs():
             assert err[key][0] < 1e-3, (k, key, err[key])
             assert err[key][1] < bound_med[key], (k, key, err[key])
     print(f"    worst max difference per field: {worst}", flush=True)
+
+
+@pytest.mark.timeout(600)
+def test_large_spheres_rigid_motion_closed_form():
+    """The large workload at full size (16M particles in four spheres, 512^3 grid,
+    dt = 2.5e-5) before any two spheres can interact (surfaces start >= 1 cm apart,
+    shared grid nodes need < 3 cells = 5.9 mm, closing speed <= 8 sqrt(3) m/s: 8
+    substeps close at most 2.8 mm): every sphere translates rigidly, so C = 0,
+    F_E = I, J_P = 1, v = v0 + g k dt and x = x0 + k dt v0 + g dt^2 k (k + 1) / 2.
+    Bounds at FP32 resolution of the state: v 1e-5 and C 1e-3 of the speed scale
+    (C per cell), x 4e-6 m, F_E and J_P 1e-5."""
+    from mpm_snow_sim2d_amd.scenes import colliding_spheres
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P = snow_params(3, 512, 2.5e-5)
+    ps = colliding_spheres(seed=0)
+    k = 8
+    print(f"[fullsize] large: {ps.n} particles, 512^3, {k} substeps", flush=True)
+    with GPUSolver(P, ps, Options(sort_every=32, tile_particles=256)) as s:
+        s.Step(k)
+        st = s.get_state()
+        assert s.check_finite() == 0
+    dt = float(np.float32(P.DELTA_TIME))
+    g = float(np.float32(P.GRAVITY))
+    v0 = ps.v.astype(np.float64)
+    grav = np.array([0.0, 1.0, 0.0])[:, None]
+    v_exact = v0 + grav * (g * k * dt)
+    x_exact = ps.x.astype(np.float64) + k * dt * v0 + grav * (g * dt * dt * k * (k + 1) / 2)
+    vmax = float(np.abs(v0).max())
+    dev = _deviation(st["x"], st["v"], st["C"], st["F"], st["Jp"], x_exact, v_exact)
+    print(f"    {dev} (speed scale {vmax:.2f} m/s)", flush=True)
+    assert dev["v"] < 1e-5 * vmax
+    assert dev["x"] < 4e-6
+    assert dev["C"] < 1e-3 * vmax / float(P.dx)
+    assert dev["F"] < 1e-5
+    assert dev["Jp"] < 1e-5
