@@ -18,8 +18,11 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libmpm_hip.so")
-SOURCES = ["mpm_kernels.hip", "mpm_fused.hip", "mpm_sort.hip", "mpm_capi.hip", "mpm_unet.hip",
-           "mpm_surrogate_net.hip", "mpm_surrogate.hip", "mpm_io.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_fused.hip", "mpm_fused_cubic.hip", "mpm_sort.hip", "mpm_capi.hip",
+           "mpm_unet.hip", "mpm_surrogate_net.hip", "mpm_surrogate.hip", "mpm_io.hip"]
+# per-source flags: the 3D cubic Coulomb-wall fused kernel unrolls its 64-node
+# stencil loops completely only above the default full-unroll limit
+EXTRA = {"mpm_fused_cubic.hip": ["-mllvm", "-pragma-unroll-threshold=65536"]}
 ARCH = os.environ.get("MPM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -53,7 +56,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
