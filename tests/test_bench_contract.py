@@ -1,0 +1,40 @@
+"""bench.py's contract pieces that run without a GPU: the CPU-baseline object
+(all-cores oracle with the one-thread rate beside it), the traffic figure read from
+the committed rocprofv3 summary, and the refusal of more than one GPU."""
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_cpu_baseline_object():
+    from mpm_snow_sim2d_amd.params import snow_params
+    from mpm_snow_sim2d_amd.scenes import snowball_disks
+    P = snow_params(2, 128, 1e-4)
+    ps = snowball_disks(seed=0)
+    out = bench.cpu_baseline(P, ps, 1, budget_s=0.2)
+    assert set(out) >= {"value", "unit", "cores", "kind", "sample", "single_thread"}
+    assert out["kind"] == "port" and out["unit"] == "M particle-substeps/s"
+    assert out["value"] > 0 and out["cores"] >= 1
+    assert out["single_thread"]["cores"] == 1 and out["single_thread"]["value"] > 0
+    json.dumps(out)
+
+
+def test_traffic_from_committed_profile():
+    t = bench.traffic_from_profiles()
+    assert t is not None
+    # the summary's traffic is the scaled FETCH_SIZE + WRITE_SIZE of the fused kernel:
+    # at least the 200 B/particle it must move, and not absurdly more
+    assert 2_000_376 * 200 <= t <= 4 * 2_000_376 * 200
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
