@@ -244,29 +244,35 @@ __device__ inline void svd2(const float* M, float* U, float* sig, float* V)
 __device__ inline void jacobi_pair(float& spp, float& sqq, float& spq, float& sr_p, float& sr_q,
                                    float* V, int p, int q)
 {
-    // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]]; the
-    // reciprocals and roots use the hardware's single-instruction approximations
-    // (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp).  A skipped rotation (c = 1,
-    // s = 0) leaves every value as it is except s_pq = 0, so it is applied as that
-    // alone: a wave whose lanes have all converged passes the pair at once.  Besides
-    // the reference's absolute test, a pair is taken as converged once
-    // |s_pq| <= 2^-24 sqrt(s_pp s_qq): the rotation would then move the singular
-    // values by less than FP32 resolves, and M = U diag(sigma) V^T with U = M V
-    // diag(1/sigma) holds for any V, so the reconstructed F_E, the rotation U V^T and
-    // the stress change only at round-off level.
+    // (c, s): symmetric Schur decomposition of [[spp, spq], [spq, sqq]] with the
+    // reference's choice of root: t = tan(theta) = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+    // tau = (s_qq - s_pp) / (2 s_pq), written without the division by 2 s_pq as
+    // |t| = |2 s_pq| / (|d| + sqrt(d^2 + 4 s_pq^2)), d = s_qq - s_pp (one reciprocal
+    // fewer); the rotation zeroes s_pq and moves the diagonal by t s_pq, the form of
+    // Golub & Van Loan, Matrix Computations, sec. 8.5.2, equal in exact arithmetic
+    // to the reference's c^2 s_pp - 2 s c s_pq + s^2 s_qq.  The reciprocals and roots
+    // use the hardware's single-instruction approximations (v_rcp_f32, v_sqrt_f32,
+    // v_rsq_f32, about 1 ulp).  A skipped rotation (c = 1, s = 0) leaves every value
+    // as it is except s_pq = 0, so it is applied as that alone: a wave whose lanes
+    // have all converged passes the pair at once.  Besides the reference's absolute
+    // test, a pair is taken as converged once |s_pq| <= 2^-24 sqrt(s_pp s_qq): the
+    // rotation would then move the singular values by less than FP32 resolves, and
+    // M = U diag(sigma) V^T with U = M V diag(1/sigma) holds for any V, so the
+    // reconstructed F_E, the rotation U V^T and the stress change only at round-off
+    // level.  (Past the absolute test |2 s_pq| >= 2e-10, so d^2 + 4 s_pq^2 cannot
+    // underflow and |d| + sqrt(...) > 0.)
     if (fabsf(spq) < 1e-10f || spq * spq <= 3.5527137e-15f * (spp * sqq)) {
         spq = 0.0f;
         return;
     }
-    float tau = (sqq - spp) * __builtin_amdgcn_rcpf(2.0f * spq);
-    float stt = __builtin_amdgcn_sqrtf(1.0f + tau * tau);
-    float t = __builtin_amdgcn_rcpf(fabsf(tau) + stt);
-    if (tau < 0.0f) t = -t;
-    const float c = __builtin_amdgcn_rsqf(1.0f + t * t);
+    const float d = sqq - spp, e2 = 2.0f * spq;
+    const float r = __builtin_amdgcn_sqrtf(fmaf(d, d, e2 * e2));
+    float t = fabsf(e2) * __builtin_amdgcn_rcpf(fabsf(d) + r);
+    if (d != 0.0f && ((d < 0.0f) != (e2 < 0.0f))) t = -t;     // tau < 0
+    const float c = __builtin_amdgcn_rsqf(fmaf(t, t, 1.0f));
     const float s = t * c;
-    float Spp = spp, Sqq = sqq, Spq = spq;
-    spp = c * c * Spp - 2.0f * s * c * Spq + s * s * Sqq;
-    sqq = s * s * Spp + 2.0f * s * c * Spq + c * c * Sqq;
+    spp = spp - t * spq;
+    sqq = sqq + t * spq;
     spq = 0.0f;
     // the remaining off-diagonal pair (r,p), (r,q) with r the third index
     float Srp = sr_p, Srq = sr_q;

@@ -25,13 +25,14 @@
 // the result never depends on how stale the binning is.
 //
 // Tile accumulation is exact fixed point.  On gfx950 the LDS float atomic retires
-// about a third of a lane per clock per CU, the 32-bit integer LDS add about twelve
+// about a third of a lane per clock per CU, the integer LDS add about twelve
 // (measured), so each contribution is scaled by a per-item power of two 2^e with
 // |value| * 2^e <= 2^21 (e from the item's largest per-particle bound), rounded to
-// an integer and summed in int32 LDS words (at most 512 contributions per node, so
-// no overflow): the tile sum is exact and independent of lane order, and the only
-// rounding is the per-contribution quantisation (<= 2^-22 of the item's largest
-// contribution bound) and the one conversion of the sum back to FP32.
+// an integer and summed exactly in LDS, two components per 64-bit add (pack_fixed;
+// at most 512 contributions per node, so no half overflows): the tile sum is exact
+// and independent of lane order, and the only rounding is the per-contribution
+// quantisation (<= 2^-22 of the item's largest contribution bound) and the one
+// conversion of the sum back to FP32.
 #include <type_traits>
 
 #include "mpm_kernels.hpp"
@@ -45,6 +46,32 @@ template <int DIM> struct TileShape {
     static constexpr int TN = DIM == 3 ? T * T * T : T * T;
     static constexpr int NB = DIM == 3 ? 27 : 9;         // grid blocks a tile overlaps
 };
+
+// Fixed-point contributions travel as the bits of fma(a, b, 1.5 * 2^23): the
+// fused multiply-add rounds once, to a float whose unit in the last place is 1, so
+// for |a * b| < 2^22 its bits are 0x4B400000 + a * b rounded to the nearest integer
+// (ties to even).  Two of them pack into one 64-bit word hi * 2^32 + lo (two's
+// complement), whose sums are exact: the 64-bit adds carry the low half's sign into
+// the high half, and both halves of a node's sum stay within 2^30.  Packing the
+// biased bits and removing both biases with one 64-bit subtraction gives that word
+// directly.
+constexpr unsigned FIXED_BIAS = 0x4B400000u;
+
+__device__ inline int fixed_bits(float a, float b) { return __float_as_int(fmaf(a, b, 12582912.0f)); }
+
+__device__ inline unsigned long long pack_fixed(int lo_bits, int hi_bits)
+{
+    constexpr unsigned long long B = ((unsigned long long)FIXED_BIAS << 32) | FIXED_BIAS;
+    return ((((unsigned long long)(unsigned)hi_bits) << 32) | (unsigned)lo_bits) - B;
+}
+
+// The two halves of a packed sum: lo is the low word as a signed int; hi follows
+// once lo's sign extension is taken back out.
+__device__ inline int unpack_lo(unsigned long long s) { return (int)(unsigned)s; }
+__device__ inline int unpack_hi(unsigned long long s)
+{
+    return (int)((long long)(s - (unsigned long long)(long long)unpack_lo(s)) >> 32);
+}
 
 template <int DIM>
 __device__ inline void tile_coords(int e, int* t)
@@ -61,22 +88,6 @@ __device__ inline bool in_grid(const Phys& ph, int i, int j, int k)
     bool ok = i >= 0 && i <= ph.W && j >= 0 && j <= ph.H;
     if constexpr (DIM == 3) ok = ok && k >= 0 && k <= ph.D;
     return ok;
-}
-
-// a * b rounded to the nearest integer (ties to even), for |a * b| < 2^22: the
-// fused multiply-add with 1.5 * 2^23 rounds once, to a float whose unit in the last
-// place is 1, and the integer sits in the low mantissa bits.
-__device__ inline int round_fixed(float a, float b)
-{
-    return __float_as_int(fmaf(a, b, 12582912.0f)) - 0x4B400000;
-}
-
-// Fixed-point add of one contribution into an int32 LDS accumulator: v * scale
-// rounded to the nearest integer (|v| * scale <= 2^21 by the choice of scale).
-__device__ inline void lds_add_fixed(int* p, float v, float scale)
-{
-    const int q = round_fixed(v, scale);
-    if (q != 0) atomicAdd(p, q);
 }
 
 // Power-of-two scale 2^e with bound * 2^e <= 2^21: an item has at most 512
@@ -145,7 +156,9 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
     constexpr int TV = G2P ? TN : 1;
     constexpr int RMAX = 2;                      // particles per thread per item
     constexpr float REACH = ORDER == 3 ? 2.f : 1.5f;   // max |x_i - x_p| / dx per axis
-    __shared__ int s_acc[DIM + 1][TN];
+    // scatter accumulators, per tile node two packed fixed-point pairs (pack_fixed):
+    // [0] (mass, p_x), [1] (p_y, p_z) — in 2D (p_y, 0)
+    __shared__ unsigned long long s_acc[2][TN];
     // gather velocities, one float4 per node: friction velocity in .xyz (2D: .xy);
     // with Coulomb walls a second float4 per node holds the collision velocity
     __shared__ float4 s_vel[COULOMB && G2P ? 2 : 1][TV];
@@ -212,8 +225,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             for (int k = 0; k < NR; ++k) {
                 const int e = ltid + 256 * k;
                 if (e >= TN) continue;
-#pragma unroll
-                for (int c = 0; c <= DIM; ++c) s_acc[c][e] = 0;
+                s_acc[0][e] = 0ull;
+                s_acc[1][e] = 0ull;
                 if constexpr (G2P) {
                     s_vel[0][e] = make_float4(va[k].y, va[k].z, va[k].w, 0.f);
                     if constexpr (COULOMB) s_vel[COULOMB ? 1 : 0][e] = vb[k];
@@ -563,8 +576,12 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             }
             if (inside) {
                 // through the tile, whole stencil inside: the inputs pre-multiplied by
-                // the item's power-of-two scales (exact), one rounding and one LDS add
-                // per component
+                // the item's power-of-two scales (exact), one rounding per component
+                // (fixed_bits) and two 64-bit LDS adds of packed component pairs per
+                // node.  Each lane walks its stencil from a start rotated by its lane
+                // index, so that lanes whose particles share a stencil base (sorted
+                // neighbours) add to different words at each step instead of
+                // serialising on one.
                 const float pms = in[r].pm * scale_m;
                 float mvs[DIM], As[DIM * DIM];
 #pragma unroll
@@ -573,35 +590,58 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 #pragma unroll
                     for (int c = 0; c < DIM; ++c) As[rr * DIM + c] = in[r].A[rr * DIM + c] * scale_p;
                 }
+                float wr[DIM][W], xr[DIM][W];
+                int er[DIM][W];
+                {
+                    const int strd[3] = {1, T, T * T};
+                    int lr = tid & 63;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const int rot = lr % W;
+                        lr /= W;
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) {
+                            const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
+                            float wv = wa[d][0], xv = xa[d][0];
+#pragma unroll
+                            for (int t2 = 1; t2 < W; ++t2)
+                                if (idx == t2) { wv = wa[d][t2]; xv = xa[d][t2]; }
+                            wr[d][s2] = wv;
+                            xr[d][s2] = xv;
+                            er[d][s2] = (rb[d] + idx) * strd[d];
+                        }
+                    }
+                }
 #pragma unroll
                 for (int a0 = 0; a0 < W; ++a0)
 #pragma unroll
                     for (int a1 = 0; a1 < W; ++a1) {
-                        const float w01 = wa[0][a0] * wa[1][a1];
+                        const float w01 = wr[0][a0] * wr[1][a1];
+                        // momentum rows up to the last axis' term (the same chain of
+                        // additions as the full sum, which continues below)
+                        float sb[DIM];
+#pragma unroll
+                        for (int rr = 0; rr < DIM; ++rr)
+                            sb[rr] = (mvs[rr] + As[rr * DIM] * xr[0][a0]) + As[rr * DIM + 1] * xr[1][a1];
 #pragma unroll
                         for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
                             float w = w01;
-                            float xi[DIM];
-                            xi[0] = xa[0][a0];
-                            xi[1] = xa[1][a1];
+                            float sm[DIM];
+#pragma unroll
+                            for (int rr = 0; rr < DIM; ++rr) sm[rr] = sb[rr];
                             if constexpr (DIM == 3) {
-                                w = w * wa[2][a2];
-                                xi[2] = xa[2][a2];
+                                w = w * wr[2][a2];
+#pragma unroll
+                                for (int rr = 0; rr < DIM; ++rr) sm[rr] += As[rr * DIM + 2] * xr[2][a2];
                             }
-                            // round to nearest, the rounding folded into the FMA
-                            int q[DIM + 1];
-                            q[0] = round_fixed(w, pms);
+                            int qb[4];
+                            qb[0] = fixed_bits(w, pms);
 #pragma unroll
-                            for (int rr = 0; rr < DIM; ++rr) {
-                                float sm = mvs[rr];
-#pragma unroll
-                                for (int c = 0; c < DIM; ++c) sm += As[rr * DIM + c] * xi[c];
-                                q[1 + rr] = round_fixed(w, sm);
-                            }
-                            const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                                   : (rb[1] + a1) * T + (rb[0] + a0);
-#pragma unroll
-                            for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc[c][e], q[c]);
+                            for (int rr = 0; rr < DIM; ++rr) qb[1 + rr] = fixed_bits(w, sm[rr]);
+                            if constexpr (DIM == 2) qb[3] = (int)FIXED_BIAS;
+                            const int e = DIM == 3 ? er[0][a0] + er[1][a1] + er[DIM - 1][a2] : er[0][a0] + er[1][a1];
+                            atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
+                            atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
                         }
                     }
                 continue;
@@ -629,9 +669,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                             if (decltype(fast)::value || via_tile) {
                                 const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                        : (rb[1] + a1) * T + (rb[0] + a0);
-                                lds_add_fixed(&s_acc[0][e], vals[0], scale_m);
+                                int qb[4];
+                                qb[0] = fixed_bits(vals[0], scale_m);
 #pragma unroll
-                                for (int c = 1; c <= DIM; ++c) lds_add_fixed(&s_acc[c][e], vals[c], scale_p);
+                                for (int c = 1; c <= DIM; ++c) qb[c] = fixed_bits(vals[c], scale_p);
+                                if constexpr (DIM == 2) qb[3] = (int)FIXED_BIAS;
+                                atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
+                                atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
                             } else {
                                 float* dstp =
                                     reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
@@ -664,7 +708,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         }
         for (int pe = tid; pe < TN * NC; pe += 256) {
             const int e = pe / NC, c = pe - e * NC;
-            const int acc = s_acc[c][e];
+            const unsigned long long sum = s_acc[c >> 1][e];
+            const int acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
             if (acc == 0) continue;
             int t[3];
             tile_coords<DIM>(e, t);
