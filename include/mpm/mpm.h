@@ -77,7 +77,10 @@ int mpm_create(const mpm_params* params, const mpm_particles* init,
 int mpm_destroy(mpm_solver* s);
 
 /* GPUSolver::Step (GPUSolver.cpp:63-80), n_substeps times, asynchronous on the
- * solver's stream. */
+ * solver's stream.  MPM_E_STATE on a solver that has taken a learned-surrogate
+ * macro step (mpm_surrogate_advance): from then on mpm_step, the stage calls,
+ * mpm_sort, mpm_reset_grid and mpm_set_state are refused, and the state stays
+ * readable (a solver created from mpm_get_state's output steps again). */
 int mpm_step(mpm_solver* s, int n_substeps);
 
 /* Stage-level drop-ins, one per dispatch of GPUSolver::Step:
@@ -260,7 +263,9 @@ int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters);
  * MPM_ADVANCE_BALLISTIC (zero increment) when the residual (share of the mass in
  * the wall band or outside the image) is <= tol, else k plain substeps;
  * MPM_ADVANCE_PLAIN always steps.  *used: the mode that ran; *residual: the test's
- * value. */
+ * value.  Plain substeps are taken only before the solver's first macro step:
+ * afterwards MPM_ADVANCE_PLAIN and a failed residual test return MPM_E_STATE (see
+ * mpm_step). */
 int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,
                           float* residual);
 

@@ -149,7 +149,9 @@ int alloc_particles(mpm_solver* s, ParticleBuffers& b)
 
 int reset_grid(mpm_solver* s, int which)
 {
-    return timed(s, MPM_KERNEL_RESET, [&] { return launch_reset(s->cfg, s->dev_grid(which), s->stream); });
+    // every scene's blocks (a batch numbers them one after another)
+    return timed(s, MPM_KERNEL_RESET,
+                 [&] { return launch_reset(s->cfg, s->dev_grid(which), s->nblocks, s->stream); });
 }
 
 int reset_all_grids(mpm_solver* s)
@@ -571,6 +573,18 @@ int reject_batch(mpm_solver* s)
     return MPM_OK;
 }
 
+// Binning, grid and substep calls on particles a learned-surrogate macro step has
+// moved are not enabled: two runs that took plain substeps after macro steps
+// ended in illegal memory accesses whose cause is not established.  The state
+// stays readable; a solver created from it (mpm_get_state, mpm_create) steps.
+int refuse_after_macro(mpm_solver* s)
+{
+    if (!s->after_macro) return MPM_OK;
+    return fail(MPM_E_STATE,
+                "plain substeps, binning and stage calls are not enabled on a solver after a surrogate macro "
+                "step; read its state out (mpm_get_state) and create a new solver from it");
+}
+
 }  // namespace
 
 extern "C" {
@@ -691,6 +705,10 @@ int mpm_step(mpm_solver* s, int n_substeps)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     if (n_substeps < 0) return fail(MPM_E_INVALID, "n_substeps < 0");
+    if (n_substeps > 0) {
+        const int r0 = refuse_after_macro(s);
+        if (r0) return r0;
+    }
     if (s->n == 0 || n_substeps == 0) return MPM_OK;
     // the fused pipeline implements the MLS scheme; the CPU path's scheme runs the
     // four stages
@@ -706,6 +724,7 @@ int mpm_p2g(mpm_solver* s)
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     int r = reject_batch(s);
     if (r) return r;
+    if ((r = refuse_after_macro(s))) return r;
     r = prepare_stage_call(s);
     if (r) return r;
     DevParticles p = s->dev_particles();
@@ -720,6 +739,7 @@ int mpm_grid_update(mpm_solver* s)
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     int r = reject_batch(s);
     if (r) return r;
+    if ((r = refuse_after_macro(s))) return r;
     r = prepare_stage_call(s);
     if (r) return r;
     DevGrid g = s->dev_grid(s->cur_grid);
@@ -733,6 +753,7 @@ int mpm_g2p(mpm_solver* s)
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     int r = reject_batch(s);
     if (r) return r;
+    if ((r = refuse_after_macro(s))) return r;
     r = prepare_stage_call(s);
     if (r) return r;
     DevParticles p = s->dev_particles();
@@ -745,12 +766,16 @@ int mpm_g2p(mpm_solver* s)
 int mpm_reset_grid(mpm_solver* s)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    const int r = refuse_after_macro(s);
+    if (r) return r;
     return reset_all_grids(s);
 }
 
 int mpm_sort(mpm_solver* s)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    const int r = refuse_after_macro(s);
+    if (r) return r;
     return do_sort(s);
 }
 
@@ -806,11 +831,12 @@ int mpm_set_state(mpm_solver* s, const float* x, const float* v, const float* C,
                   const float* Jp)
 {
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    int r = refuse_after_macro(s);
+    if (r) return r;
     const int n = s->n, d = s->dim;
     if (n == 0) return MPM_OK;
     const ParticleBuffers& b = s->pb[s->cur_pb];
     std::vector<uint32_t> pid(n);
-    int r;
     if ((r = copy_to_host(s, pid.data(), b.pid, (size_t)n * 4))) return r;
     HIP_TRY(hipStreamSynchronize(s->stream));
     struct { const float* in; float* dst; int comps; } f[] = {

@@ -545,9 +545,10 @@ hipError_t launch_g2p(const LaunchCfg& c, const DevParticles& p, const DevGrid& 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_reset(const LaunchCfg& c, const DevGrid& g, hipStream_t st)
+hipError_t launch_reset(const LaunchCfg& c, const DevGrid& g, int nblocks, hipStream_t st)
 {
-    dim3 grid(nblocks_total(g)), blk(64);
+    if (nblocks <= 0) return hipSuccess;
+    dim3 grid(nblocks), blk(64);
     if (c.dim == 2) k_reset<2><<<grid, blk, 0, st>>>(g);
     else            k_reset<3><<<grid, blk, 0, st>>>(g);
     return hipGetLastError();

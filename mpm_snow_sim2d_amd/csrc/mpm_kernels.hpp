@@ -19,7 +19,8 @@ hipError_t launch_grid_update(const LaunchCfg& c, const DevGrid& g, const Phys& 
                               hipStream_t st);
 hipError_t launch_g2p(const LaunchCfg& c, const DevParticles& p, const DevGrid& g,
                       const Phys& ph, hipStream_t st);
-hipError_t launch_reset(const LaunchCfg& c, const DevGrid& g, hipStream_t st);
+// nblocks: the blocks to scan — every scene's, one after another, for a batch.
+hipError_t launch_reset(const LaunchCfg& c, const DevGrid& g, int nblocks, hipStream_t st);
 
 // Grid update of `cur` fused with the clear of `old` (the grid the previous G2P
 // read): both driven by the block flags, no dense sweep.

@@ -70,6 +70,9 @@ struct mpm_solver {
     bool sorted_keys_in_alt = false;
     int since_sort = 0;
     int64_t substeps_done = 0;   // substeps advanced since creation (snapshots)
+    // a learned-surrogate macro step has moved the particles (mpm_surrogate_advance):
+    // binning, grid and substep calls are refused from then on (MPM_E_STATE)
+    bool after_macro = false;
 
     unsigned long long* d_bad = nullptr;
 

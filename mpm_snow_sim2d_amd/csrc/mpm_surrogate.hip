@@ -1001,6 +1001,10 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
         // shared buffers (capture, network)
         S_TRY(hipEventRecord(g->ev_capture, st));
         S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
+        // plain substeps after an earlier macro step are refused by mpm_step
+        if (s->after_macro)
+            return sfail(MPM_E_STATE, "the residual test failed after an earlier macro step, and plain "
+                                      "substeps after a macro step are not enabled");
         return mpm_step(s, k_substeps);
     }
 
@@ -1019,6 +1023,7 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     // the solver's grid and binning no longer match its particles
     s->grid_state = mpm::detail::GRID_DIRTY;
     s->items_valid = false;
+    s->after_macro = true;
     s->substeps_done += k_substeps;
     if (used) *used = mode;
     return MPM_OK;

@@ -209,8 +209,13 @@ def falling_cube(seed: int = 0, side: float = 0.3, n_side: int = 126, bottom: fl
 def colliding_spheres(seed: int = 0, n_target: int = 16_000_000, count: int = 4, rho: float = 400.0,
                       per_cell_dx: float = 1.0 / 512) -> ParticleSet:
     """Plain 3D large: `count` spheres of radius U(0.08, 0.15) at seeded
-    non-overlapping centres in [0.2, 0.8]^3, velocities U(-4, 4) m/s per component,
-    jitter-sampled so the total is about n_target particles."""
+    non-overlapping centres in [0.2, 0.8]^3, each moving towards the spheres'
+    common centroid at U(2, 4) m/s plus a U(-0.5, 0.5) m/s perturbation per
+    component (components within +-4 m/s), so that they collide; jitter-sampled so
+    the total is about n_target particles.  meta holds the centres, radii and
+    velocities.  Seed 2 (LARGE_SEED) brings the first two spheres into contact
+    after ~157 substeps of 2.5e-5 s and three pairs within 1000, all far from the
+    walls."""
     rng = np.random.default_rng(seed)
     spheres = []
     while len(spheres) < count:
@@ -218,15 +223,25 @@ def colliding_spheres(seed: int = 0, n_target: int = 16_000_000, count: int = 4,
         c = rng.uniform(0.2, 0.8, 3)
         if all(np.linalg.norm(c - c2) > r + r2 + 0.01 for c2, r2 in spheres):
             spheres.append((c, r))
+    mid = np.mean([c for c, _ in spheres], axis=0)
+    vels = []
+    for c, _ in spheres:
+        d = (mid - c) / max(float(np.linalg.norm(mid - c)), 1e-9)
+        vels.append(np.clip(rng.uniform(2.0, 4.0) * d + rng.uniform(-0.5, 0.5, 3), -4.0, 4.0))
     total_vol = sum(4.0 / 3.0 * math.pi * r ** 3 for _, r in spheres)
     h = (total_vol / n_target) ** (1.0 / 3.0)
     out = ParticleSet.empty(3)
-    for c, r in spheres:
-        v = rng.uniform(-4.0, 4.0, 3)
+    for (c, r), v in zip(spheres, vels):
         pts = jitter_lattice(c - r, c + r, h, rng, lambda p, c=c, r=r: ((p - c) ** 2).sum(1) <= r * r)
         out = out.concat(_set(pts, v, h ** 3, rho * h ** 3))
-    out.meta = {"scene": "colliding_spheres", "seed": seed, "h": h}
+    out.meta = {"scene": "colliding_spheres", "seed": seed, "h": h,
+                "centres": [c.tolist() for c, _ in spheres], "radii": [float(r) for _, r in spheres],
+                "velocities": [v.tolist() for v in vels]}
     return out
+
+
+# the large workload's seed: its spheres meet within the configured 1000 substeps
+LARGE_SEED = 2
 
 
 def random_scene(seed: int, n_target: int, rho: float = 400.0, max_shapes: int = 3) -> ParticleSet:

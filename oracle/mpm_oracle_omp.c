@@ -60,14 +60,38 @@ static void zero_par(float* a, size_t n)
     }
 }
 
-/* nsteps substeps of the MLS scheme (p2g -> grid update -> g2p -> reset), the layout
- * and grid scratch of orc_run_f32. */
+/* nsteps substeps (p2g -> grid update -> g2p -> reset) of the MLS scheme, or of the
+ * CPU path's APIC scheme with the explicit grid force (src/cpu/Particles.cpp:115-131,
+ * 2D), with the layout and grid scratch of orc_run_f32 (the force grid is
+ * allocated here). */
 void orc_run_omp_f32(const mpm_params* p, long n, int nsteps, float* x, float* v, const float* mass,
                      const float* vol, float* B, float* Fe, float* Fp, float* gmass, float* gmom, float* gvcol,
                      float* gvfric)
 {
     const long G = orc_num_nodes(p);
     const size_t vn = (size_t)p->dim * G;
+    if (p->scheme == MPM_SCHEME_APIC_FORCE) {
+        float* gforce = (float*)calloc(vn, sizeof(float));
+        if (!gforce) return;
+        for (int s = 0; s < nsteps; ++s) {
+            if (p->dim == 3) {
+                p2g_apic_omp_3(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom, gforce);
+                grid_update_apic_omp_3(p, gmass, gmom, gforce, gvcol, gvfric);
+                g2p_apic_omp_3(p, n, x, v, B, Fe, Fp, gvcol, gvfric);
+            } else {
+                p2g_apic_omp_2(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom, gforce);
+                grid_update_apic_omp_2(p, gmass, gmom, gforce, gvcol, gvfric);
+                g2p_apic_omp_2(p, n, x, v, B, Fe, Fp, gvcol, gvfric);
+            }
+            zero_par(gmass, (size_t)G);
+            zero_par(gmom, vn);
+            zero_par(gforce, vn);
+            zero_par(gvcol, vn);
+            zero_par(gvfric, vn);
+        }
+        free(gforce);
+        return;
+    }
     for (int s = 0; s < nsteps; ++s) {
         if (p->dim == 3) {
             p2g_omp_3(p, n, x, v, mass, vol, B, Fe, Fp, gmass, gmom);

@@ -245,11 +245,10 @@ def omp_threads() -> int:
 
 
 def run_omp(params, st: OracleState, nsteps: int) -> OracleState:
-    """nsteps FP32 substeps of the MLS scheme on every host core."""
+    """nsteps FP32 substeps on every host core: the MLS scheme, or the CPU path's
+    APIC scheme with the explicit grid force (params.scheme)."""
     if st.dtype != np.float32:
         raise ValueError("the multi-threaded oracle runs in FP32")
-    if getattr(params, "scheme", 0) != 0:
-        raise ValueError("the multi-threaded oracle runs the MLS scheme only")
     lib = load_omp()
     G = lib.orc_num_nodes(ctypes.byref(params))
     d = st.dim

@@ -23,6 +23,12 @@ def test_cpu_baseline_object():
     assert out["value"] > 0 and out["cores"] >= 1
     assert out["single_thread"]["cores"] == 1 and out["single_thread"]["value"] > 0
     json.dumps(out)
+    # the reference CPU path's own algorithm (APIC + explicit grid force) as the
+    # reported legs, the MLS restatement beside them
+    a = bench.cpu_baseline(P, ps, 1, budget_s=0.2, apic=True)
+    assert a["value"] > 0 and a["single_thread"]["value"] > 0 and a["single_thread"]["cores"] == 1
+    assert "APIC" in a["sample"] and a["glsl_mls"]["value"] > 0
+    json.dumps(a)
 
 
 def test_traffic_from_committed_profile():

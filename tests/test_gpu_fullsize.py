@@ -110,10 +110,10 @@ def test_large_spheres_rigid_motion_closed_form():
     F_E = I, J_P = 1, v = v0 + g k dt and x = x0 + k dt v0 + g dt^2 k (k + 1) / 2.
     Bounds at FP32 resolution of the state: v 1e-5 and C 1e-3 of the speed scale
     (C per cell), x 4e-6 m, F_E and J_P 1e-5."""
-    from mpm_snow_sim2d_amd.scenes import colliding_spheres
+    from mpm_snow_sim2d_amd.scenes import LARGE_SEED, colliding_spheres
     from mpm_snow_sim2d_amd.solver import GPUSolver
     P = snow_params(3, 512, 2.5e-5)
-    ps = colliding_spheres(seed=0)
+    ps = colliding_spheres(seed=LARGE_SEED)
     k = 8
     print(f"[fullsize] large: {ps.n} particles, 512^3, {k} substeps", flush=True)
     with GPUSolver(P, ps, Options(sort_every=32, tile_particles=256)) as s:

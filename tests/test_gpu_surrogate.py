@@ -409,6 +409,55 @@ def test_advance_falls_back_at_walls():
     assert np.abs(sa["x"] - sb["x"]).max() < 1e-6
 
 
+def test_plain_substeps_after_macro_step_are_refused():
+    """After a macro step the solver refuses plain substeps (mpm_step, a PLAIN
+    advance, a failed residual test), binning, grid resets and state uploads with
+    MPM_E_STATE, and it runs no kernel for them; its state stays readable, and a
+    solver created from that state steps like one created from the same arrays."""
+    from mpm_snow_sim2d_amd._lib import MPM_E_STATE, MPMError
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    from mpm_snow_sim2d_amd.surrogate import ADVANCE_BALLISTIC, ADVANCE_PLAIN, Surrogate
+    P = snow_params(2, 64, 2e-4)
+    rng = np.random.default_rng(6)
+    c = np.array([0.5, 0.55])
+    pts = jitter_lattice(c - 0.08, c + 0.08, 0.004, rng, lambda p: ((p - c) ** 2).sum(1) <= 0.08 ** 2)
+    ps = _set(pts, (0.5, -0.2), 0.004 ** 2, 400 * 0.004 ** 2)
+    k = 8
+    with Surrogate(64, 64, max_batch=1, capacity=0) as sur:
+        s = GPUSolver(P, ps, Options(sort_every=4))
+        used, _ = sur.advance(s, k, ADVANCE_BALLISTIC, 1.0)
+        assert used == ADVANCE_BALLISTIC
+        st = s.get_state()
+        refused = [lambda: s.Step(1), lambda: sur.advance(s, k, ADVANCE_PLAIN, 1.0),
+                   lambda: sur.advance(s, k, ADVANCE_BALLISTIC, -1.0),     # residual test fails
+                   s.sort, s.reset, s.p2g, lambda: s.set_state(**st)]
+        for call in refused:
+            with pytest.raises(MPMError) as err:
+                call()
+            assert err.value.code == MPM_E_STATE
+        st2 = s.get_state()
+        for key in st:
+            assert np.array_equal(st[key], st2[key]), key
+        # a further macro step is still taken
+        used, _ = sur.advance(s, k, ADVANCE_BALLISTIC, 1.0)
+        assert used == ADVANCE_BALLISTIC
+        s.close()
+    # the way on: a new solver from the read-out state
+    moved = ParticleSet(st["x"], st["v"], ps.mass, ps.vol)
+    outs = []
+    for _ in range(2):
+        with GPUSolver(P, moved, Options(sort_every=4), C=st["C"], F=st["F"], Jp=st["Jp"]) as t:
+            t.Step(k)
+            outs.append(t.get_state())
+            assert t.check_finite() == 0
+    # the two runs differ only in the order of float atomics at shared tile-halo
+    # nodes: natural units (C against max|v| / dx), the batch tests' bounds
+    from tests.helpers import state_errors
+    err = state_errors(outs[0], outs[1], float(P.dx))
+    for key, (mx, med) in err.items():
+        assert mx < 1e-3 and med < 1e-5, (key, mx, med)
+
+
 def test_trained_network_beats_ballistic():
     """Ablation: train on trajectories of colliding scenes (64^2 grid), then on
     held-out scenes compare the network with the zero-output (ballistic) method:

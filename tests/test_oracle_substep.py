@@ -271,12 +271,16 @@ def test_f32_oracle_tracks_f64(orc, dim):
     assert np.abs(s32.Fe - s64.Fe).max() < 1e-4
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_multithreaded_baseline_matches_serial_oracle(orc, dim):
+@pytest.mark.parametrize("dim,scheme", [(2, "mls"), (3, "mls"), (2, "apic")])
+def test_multithreaded_baseline_matches_serial_oracle(orc, dim, scheme):
     """The all-cores CPU baseline (OpenMP build of the same body) runs the same
-    substep: after 10 substeps it agrees with the serial FP32 oracle up to the order
-    of the grid's float sums."""
+    substep, of the GLSL path's MLS scheme and of the CPU path's APIC scheme with
+    the explicit grid force: after 10 substeps it agrees with the serial FP32
+    oracle up to the order of the grid's float sums."""
+    from mpm_snow_sim2d_amd.params import SCHEME_APIC_FORCE
     P = snow_params(dim, 32, 1e-4)
+    if scheme == "apic":
+        P.scheme = SCHEME_APIC_FORCE
     rng = np.random.default_rng(6)
     a = _block(dim, [0.3] * dim, [0.42] * dim, 0.012, [2.0, 0.0, 0.0][:dim], rng)
     b = _block(dim, [0.44] + [0.32] * (dim - 1), [0.56] + [0.44] * (dim - 1), 0.012, [-2.0, 0.0, 0.0][:dim], rng)
