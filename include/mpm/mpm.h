@@ -310,6 +310,18 @@ int mpm_write_png(const char* path, int width, int height, const unsigned char* 
 int mpm_sort_pairs_host(const uint32_t* keys, int64_t n, int key_bits,
                         uint32_t* out_keys, uint32_t* out_perm);
 
+/* No counterpart in the reference: mpm_sort_pairs_host for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors; it calls no hipMalloc, hipFree or stream
+ * creation: its device keys, values and radix scratch live in a fixed device array
+ * of the library (about 17 MB, held on each GPU it runs on until the process ends)
+ * and it runs on the null stream.  n above MPM_SORT_NOALLOC_MAX_KEYS returns
+ * MPM_E_INVALID.  Calls from several host threads are serialised; every call
+ * overwrites the array. */
+#define MPM_SORT_NOALLOC_MAX_KEYS (1 << 20)
+int mpm_sort_pairs_host_noalloc(const uint32_t* keys, int64_t n, int key_bits,
+                                uint32_t* out_keys, uint32_t* out_perm);
+
 const char* mpm_last_error(void);
 
 #ifdef __cplusplus

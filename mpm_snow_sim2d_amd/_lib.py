@@ -31,7 +31,7 @@ EXPORTS = [
     "mpm_get_state", "mpm_set_state", "mpm_get_grid", "mpm_get_grid_force",
     "mpm_num_particles", "mpm_num_nodes",
     "mpm_set_timing", "mpm_set_timing_mask", "mpm_reset_timers", "mpm_get_kernel_time",
-    "mpm_check_finite", "mpm_sort_pairs_host", "mpm_last_error",
+    "mpm_check_finite", "mpm_sort_pairs_host", "mpm_sort_pairs_host_noalloc", "mpm_last_error",
     "mpm_batch_create", "mpm_num_scenes", "mpm_scene_range", "mpm_set_graph", "mpm_autotune",
     "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
     "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_set_conv", "mpm_surrogate_set_macro", "mpm_surrogate_set_targets", "mpm_surrogate_forward", "mpm_surrogate_gradient",
@@ -106,6 +106,7 @@ def load() -> ctypes.CDLL:
         "mpm_get_kernel_time": (i32, [vp, i32, P(ctypes.c_double), P(i64)]),
         "mpm_check_finite": (i32, [vp, P(i64)]),
         "mpm_sort_pairs_host": (i32, [vp, i64, i32, vp, vp]),
+        "mpm_sort_pairs_host_noalloc": (i32, [vp, i64, i32, vp, vp]),
         "mpm_last_error": (ctypes.c_char_p, []),
         "mpm_batch_create": (i32, [i32, P(Params), P(CParticles), P(COptions), P(vp)]),
         "mpm_num_scenes": (i32, [vp, P(i32)]),

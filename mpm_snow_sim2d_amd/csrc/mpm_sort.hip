@@ -20,9 +20,6 @@
 
 namespace mpm {
 
-static constexpr int RS_THREADS = 256;
-static constexpr int RS_ROUNDS = 16;
-static constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;   // 4096 keys
 static constexpr int RS_WAVES = RS_THREADS / 64;
 
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint32_t* keys, int n, int shift,
@@ -43,9 +40,6 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint32_t* keys, in
 
 // Exclusive scan, three kernels: per-block scan of 1024 entries, scan of the block
 // sums (one workgroup, any count), add-back.
-static constexpr int SCAN_THREADS = 256;
-static constexpr int SCAN_PER = 4;
-static constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_PER;   // 1024
 
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total)
 {
@@ -183,14 +177,6 @@ static hipError_t exclusive_scan(uint32_t* data, int n, uint32_t* sums, hipStrea
     k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(sums, nb);
     k_scan_add<<<nb, SCAN_THREADS, 0, st>>>(data, n, sums);
     return hipGetLastError();
-}
-
-size_t radix_scratch_words(int n)
-{
-    size_t ntiles = ((size_t)n + RS_TILE - 1) / RS_TILE;
-    size_t counts = 256 * ntiles;
-    size_t sums = (counts + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
-    return counts + sums + 64;
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,

@@ -8,8 +8,24 @@
 
 namespace mpm {
 
-// uint32 words of scratch radix_sort_pairs needs for n keys.
-size_t radix_scratch_words(int n);
+// Radix-sort tiling: 256-thread workgroups of 16 rounds (4096 keys per tile), and
+// the block of the exclusive scan (1024 entries).
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_PER = 4;
+constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_PER;
+
+// uint32 words of scratch radix_sort_pairs needs for n keys: per-tile digit counts,
+// the scan's block sums, and slack.
+constexpr size_t radix_scratch_words(int n)
+{
+    const size_t ntiles = ((size_t)n + RS_TILE - 1) / RS_TILE;
+    const size_t counts = 256 * ntiles;
+    const size_t sums = (counts + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
+    return counts + sums + 64;
+}
 
 // Stable LSD radix sort of n (key, val) pairs over the low key_bits bits.  Ping-pongs
 // between (keys, vals) and (keys_alt, vals_alt); *result_in_alt says where the
