@@ -156,14 +156,49 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
     constexpr int TV = G2P ? TN : 1;
     constexpr int RMAX = 2;                      // particles per thread per item
     constexpr float REACH = ORDER == 3 ? 2.f : 1.5f;   // max |x_i - x_p| / dx per axis
-    // scatter accumulators, per tile node two packed fixed-point pairs (pack_fixed):
-    // [0] (mass, p_x), [1] (p_y, p_z) — in 2D (p_y, 0)
-    __shared__ unsigned long long s_acc[2][TN];
-    // gather velocities, one float4 per node: friction velocity in .xyz (2D: .xy);
-    // with Coulomb walls a second float4 per node holds the collision velocity
-    __shared__ float4 s_vel[COULOMB && G2P ? 2 : 1][TV];
+    // Scatter accumulators.  3D: per tile node two packed fixed-point pairs
+    // (pack_fixed), [0] (mass, p_x) and [1] (p_y, p_z): one 64-bit LDS add carries two
+    // of the four components.  2D: one 32-bit word per component, [c][node]: three
+    // components pack into 64-bit pairs only with a wasted half, which measured
+    // slower there (81.5 against 88.9 us per launch on the 64-scene batch).
+    constexpr bool PACKED = DIM == 3;
+    __shared__ unsigned long long s_acc_raw[PACKED ? 2 * TN : ((DIM + 1) * TN + 1) / 2];
+    unsigned long long (*s_acc)[TN] = reinterpret_cast<unsigned long long (*)[TN]>(s_acc_raw);
+    int (*s_acc32)[TN] = reinterpret_cast<int (*)[TN]>(s_acc_raw);
+    // gather velocities.  3D: component-major, friction velocity in rows 0..2 and,
+    // with Coulomb walls, the collision velocity in rows 3..5: one float per row and
+    // node, rather than a float4 per node, whose 16-byte stride maps a node's bank to
+    // its tile index mod 16, spreads a wave's reads over the LDS banks (falling cube:
+    // ~198 -> ~195 us per launch, mean of three runs).  2D: a float4 per node,
+    // friction velocity in .xy, collision velocity in a second float4 with Coulomb
+    // walls (the component-major rows measured no faster there).
+    constexpr bool VSOA = DIM == 3;
+    constexpr int NVR = VSOA ? (COULOMB ? 2 * DIM : DIM) : 1;
+    __shared__ float s_vel[G2P && VSOA ? NVR : 1][G2P && VSOA ? TV : 1];
+    __shared__ float4 s_vel4[COULOMB && G2P && !VSOA ? 2 : 1][G2P && !VSOA ? TV : 1];
     __shared__ int s_touch[TS::NB];
     __shared__ float s_red[2][4];
+
+    // friction (vF) and collision (vC) velocity of tile node e from the staged tile
+    auto load_node_velocity = [&](int e, float* vF, float* vC) {
+        if constexpr (VSOA) {
+            vF[2] = vC[2] = 0.f;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                vF[d] = s_vel[d][e];
+                vC[d] = COULOMB ? s_vel[(DIM + d) % NVR][e] : vF[d];
+            }
+        } else {
+            const float4 f4 = s_vel4[0][e];
+            vF[0] = f4.x; vF[1] = f4.y; vF[2] = f4.z;
+            if constexpr (COULOMB) {
+                const float4 c4 = s_vel4[COULOMB ? 1 : 0][e];
+                vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+            } else {
+                vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
+            }
+        }
+    };
 
     const int n = P.n;
     const int tid = threadIdx.x;
@@ -225,11 +260,23 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             for (int k = 0; k < NR; ++k) {
                 const int e = ltid + 256 * k;
                 if (e >= TN) continue;
-                s_acc[0][e] = 0ull;
-                s_acc[1][e] = 0ull;
-                if constexpr (G2P) {
-                    s_vel[0][e] = make_float4(va[k].y, va[k].z, va[k].w, 0.f);
-                    if constexpr (COULOMB) s_vel[COULOMB ? 1 : 0][e] = vb[k];
+                if constexpr (PACKED) {
+                    s_acc[0][e] = 0ull;
+                    s_acc[1][e] = 0ull;
+                } else {
+#pragma unroll
+                    for (int c = 0; c <= DIM; ++c) s_acc32[c][e] = 0;
+                }
+                if constexpr (G2P && VSOA) {
+                    const float f[3] = {va[k].y, va[k].z, va[k].w}, c[3] = {vb[k].x, vb[k].y, vb[k].z};
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        s_vel[d][e] = f[d];
+                        if constexpr (COULOMB) s_vel[(DIM + d) % NVR][e] = c[d];
+                    }
+                } else if constexpr (G2P) {
+                    s_vel4[0][e] = make_float4(va[k].y, va[k].z, va[k].w, 0.f);
+                    if constexpr (COULOMB) s_vel4[COULOMB ? 1 : 0][e] = vb[k];
                 }
             }
         }
@@ -317,14 +364,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 if (decltype(fast)::value || tile_ok) {
                                     int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                      : (rb[1] + a1) * T + (rb[0] + a0);
-                                    const float4 f4 = s_vel[0][e];
-                                    vF[0] = f4.x; vF[1] = f4.y; vF[2] = f4.z;
-                                    if constexpr (COULOMB) {
-                                        const float4 c4 = s_vel[COULOMB ? 1 : 0][e];
-                                        vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
-                                    } else {
-                                        vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
-                                    }
+                                    load_node_velocity(e, vF, vC);
                                 } else {
                                     int addr = node_addr<DIM>(src, idx[0], idx[1], idx[2]);
                                     float4 a4 = src.a[addr];
@@ -377,13 +417,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                             for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
                                 const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                        : (rb[1] + a1) * T + (rb[0] + a0);
-                                const float4 f4 = s_vel[0][e];
-                                const float vF[3] = {f4.x, f4.y, f4.z};
-                                float vC[3] = {vF[0], vF[1], vF[2]};
-                                if constexpr (COULOMB) {
-                                    const float4 c4 = s_vel[COULOMB ? 1 : 0][e];
-                                    vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
-                                }
+                                float vF[3], vC[3];
+                                load_node_velocity(e, vF, vC);
                                 if constexpr (DIM == 3) {
                                     const float w2 = st.w[DIM - 1][a2], wd2 = wd[DIM - 1][a2];
 #pragma unroll
@@ -577,11 +612,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             if (inside) {
                 // through the tile, whole stencil inside: the inputs pre-multiplied by
                 // the item's power-of-two scales (exact), one rounding per component
-                // (fixed_bits) and two 64-bit LDS adds of packed component pairs per
-                // node.  Each lane walks its stencil from a start rotated by its lane
-                // index, so that lanes whose particles share a stencil base (sorted
-                // neighbours) add to different words at each step instead of
-                // serialising on one.
+                // (fixed_bits) and LDS adds (3D: two of packed component pairs per
+                // node).  In 3D each lane walks its stencil from a start rotated by its
+                // lane index, so that lanes whose particles share a stencil base
+                // (sorted neighbours: a 3D pass holds consecutive particles) add to
+                // different words at each step instead of serialising on one; 2D
+                // passes spread their lanes over the block's cells already, and the
+                // rotation measured slower there (81.5 against 88.3 us on the batch).
                 const float pms = in[r].pm * scale_m;
                 float mvs[DIM], As[DIM * DIM];
 #pragma unroll
@@ -594,7 +631,11 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 int er[DIM][W];
                 {
                     const int strd[3] = {1, T, T * T};
-                    int lr = tid & 63;
+                    // the lane's rotation, from an opaque copy of its index: derived per
+                    // particle (a few integer operations) instead of being hoisted out
+                    // of the item loop as compare masks kept live (or spilled) across G2P
+                    int lr = DIM == 3 ? tid & 63 : 0;
+                    asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
                         const int rot = lr % W;
@@ -638,10 +679,14 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                             qb[0] = fixed_bits(w, pms);
 #pragma unroll
                             for (int rr = 0; rr < DIM; ++rr) qb[1 + rr] = fixed_bits(w, sm[rr]);
-                            if constexpr (DIM == 2) qb[3] = (int)FIXED_BIAS;
                             const int e = DIM == 3 ? er[0][a0] + er[1][a1] + er[DIM - 1][a2] : er[0][a0] + er[1][a1];
-                            atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
-                            atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
+                            if constexpr (PACKED) {
+                                atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
+                                atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
+                            } else {
+#pragma unroll
+                                for (int c = 0; c <= DIM; ++c) atomicAdd(&s_acc32[c][e], qb[c] - (int)FIXED_BIAS);
+                            }
                         }
                     }
                 continue;
@@ -673,9 +718,16 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 qb[0] = fixed_bits(vals[0], scale_m);
 #pragma unroll
                                 for (int c = 1; c <= DIM; ++c) qb[c] = fixed_bits(vals[c], scale_p);
-                                if constexpr (DIM == 2) qb[3] = (int)FIXED_BIAS;
-                                atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
-                                atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
+                                if constexpr (PACKED) {
+                                    atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
+                                    atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
+                                } else {
+#pragma unroll
+                                    for (int c = 0; c <= DIM; ++c) {
+                                        const int q = qb[c] - (int)FIXED_BIAS;
+                                        if (q != 0) atomicAdd(&s_acc32[c][e], q);
+                                    }
+                                }
                             } else {
                                 float* dstp =
                                     reinterpret_cast<float*>(&dst.a[node_addr<DIM>(dst, idx[0], idx[1], idx[2])]);
@@ -708,8 +760,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         }
         for (int pe = tid; pe < TN * NC; pe += 256) {
             const int e = pe / NC, c = pe - e * NC;
-            const unsigned long long sum = s_acc[c >> 1][e];
-            const int acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
+            int acc;
+            if constexpr (PACKED) {
+                const unsigned long long sum = s_acc[c >> 1][e];
+                acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
+            } else {
+                acc = s_acc32[c][e];
+            }
             if (acc == 0) continue;
             int t[3];
             tile_coords<DIM>(e, t);
