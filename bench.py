@@ -139,39 +139,63 @@ def learned(args) -> dict:
     the residual test fails) and with the plain solver alone."""
     from mpm_snow_sim2d_amd.params import Options, snow_params
     from mpm_snow_sim2d_amd.scenes import colliding_scene
+    from mpm_snow_sim2d_amd.solver import autotune
     from mpm_snow_sim2d_amd.surrogate import (ADVANCE_BALLISTIC, ADVANCE_NETWORK, ADVANCE_PLAIN, Surrogate, fit,
-                                              generate, rollout)
+                                              generate, rollout, train_streaming)
 
     cells, dt, k, npart = 256, 5e-5, 8, 60_000
     P = snow_params(2, cells, dt)
     ntr, nte = args.train_scenes, args.test_scenes
     per = args.gen_substeps // k
     t0 = time.perf_counter()
-    train = [(P, colliding_scene(1000 + i, npart)) for i in range(ntr)]
     test = [(P, colliding_scene(5000 + i, npart)) for i in range(nte)]
-    log(f"[learned] scenes ready ({time.perf_counter() - t0:.1f} s)")
-    sur = Surrogate(cells, cells, max_batch=args.batch, capacity=(ntr + nte) * per, seed=0, d_ref=0.1)
+    # streamed training selects its parameters on validation scenes of their own
+    nva = nte if args.stream_chunks > 0 else 0
+    sur = Surrogate(cells, cells, max_batch=args.batch, capacity=(ntr + nte + nva) * per, seed=0, d_ref=0.1)
     sur.set_macro(args.macro)
     sur.set_targets(args.refine)
-    t0 = time.perf_counter()
-    tr = generate(sur, train, args.gen_substeps, k, wall_tol=args.tol)
     te = generate(sur, test, args.gen_substeps, k, first_slot=ntr * per, wall_tol=args.tol)
-    gen_s = time.perf_counter() - t0
-    tr_slots = [x for ss in tr for x in ss]
     te_slots = [x for ss in te for x in ss]
-    log(f"[learned] {len(tr_slots)} training / {len(te_slots)} held-out samples in {gen_s:.1f} s")
+    va = generate(sur, [(P, colliding_scene(7000 + i, npart)) for i in range(nva)], args.gen_substeps, k,
+                  first_slot=(ntr + nte) * per, wall_tol=args.tol) if nva else []
+    va_slots = [x for ss in va for x in ss]
 
     def mean_loss(fn, slots):
         return float(np.mean([fn(slots[i:i + args.batch]) for i in range(0, len(slots), args.batch)]))
 
     base_loss = mean_loss(sur.baseline_loss, te_slots)
-    t0 = time.perf_counter()
-    fit(sur, tr_slots, steps=args.train_steps, batch=args.batch, lr=args.lr, log_every=max(args.train_steps // 10, 1),
-        log=log)
-    train_s = time.perf_counter() - t0
+    if args.stream_chunks > 0:
+        # the configured data scale, streamed: chunks of ntr new trajectories each,
+        # generated into the training slots and trained on before the next chunk
+        steps = args.train_steps // (args.stream_chunks * args.stream_epochs)
+        st = train_streaming(sur, lambda c: [(P, colliding_scene(10_000 + c * ntr + i, npart)) for i in range(ntr)],
+                             args.stream_chunks, args.gen_substeps, k, first_slot=0, steps_per_chunk=steps,
+                             batch=args.batch, lr=args.lr, wall_tol=args.tol,
+                             eval_fn=lambda m: mean_loss(m.loss, va_slots), epochs=args.stream_epochs,
+                             patience=args.patience, log=log)
+        n_train, train_steps, gen_s, train_s = st["samples"], st["steps"], st["generate_s"], st["train_s"]
+        trajectories, curve = st["trajectories"], st["eval"]
+    else:
+        train = [(P, colliding_scene(1000 + i, npart)) for i in range(ntr)]
+        t0 = time.perf_counter()
+        tr = generate(sur, train, args.gen_substeps, k, wall_tol=args.tol)
+        gen_s = time.perf_counter() - t0
+        tr_slots = [x for ss in tr for x in ss]
+        n_train, train_steps, trajectories, curve = len(tr_slots), args.train_steps, ntr, []
+        log(f"[learned] {len(tr_slots)} training / {len(te_slots)} held-out samples in {gen_s:.1f} s")
+        t0 = time.perf_counter()
+        fit(sur, tr_slots, steps=args.train_steps, batch=args.batch, lr=args.lr,
+            log_every=max(args.train_steps // 10, 1), log=log)
+        train_s = time.perf_counter() - t0
     val_loss = mean_loss(sur.loss, te_slots)
-    log(f"[learned] trained {args.train_steps} steps in {train_s:.1f} s; held-out loss {val_loss:.4f} "
-        f"(ballistic {base_loss:.4f})")
+    log(f"[learned] trained {train_steps} steps on {n_train} samples of {trajectories} trajectories in "
+        f"{train_s:.1f} s; held-out loss {val_loss:.4f} (ballistic {base_loss:.4f})")
+    # the plain solver it is compared with runs the autotuner's options (binning
+    # period and work-item size) with graph replay
+    opts_plain, tuned_ms = autotune([test[0]], trial_substeps=64, graph=True)
+    log(f"[learned] plain solver options {opts_plain} ({tuned_ms:.4f} ms per substep in the tuning trial)")
+    floor_opts = Options(sort_every=32 if opts_plain.sort_every == 16 else 16,
+                         tile_particles=128 if opts_plain.tile_particles == 256 else 256)
 
     # a macro step runs the network on 1 image (velocity), or 2 with the
     # displacement; a training step on as many per slot
@@ -188,22 +212,22 @@ def learned(args) -> dict:
                 if v != "" and int(v) != args.macro and (args.macro or int(v) == 0)]
     var_res = {fl: [] for fl in variants}
     for i, (Pq, ps) in enumerate(test):
-        ref = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN)
+        ref = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN, options=opts_plain)
         res[ADVANCE_PLAIN].append({"wall": ref["wall"], "err": 0.0, "counts": ref["counts"]})
-        # the same plain rollout binned every 16 substeps instead of 8: only the order
-        # of the float additions differs, so this is the divergence floor of the scene
-        ref2 = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN, options=Options(sort_every=16, tile_particles=256))
+        # the same plain rollout with other binning options: only the order of the
+        # float additions differs, so this is the divergence floor of the scene
+        ref2 = rollout(None, Pq, ps, args.rollout, k, ADVANCE_PLAIN, options=floor_opts)
         floor.append(float(np.linalg.norm(ref2["x"] - ref["x"], axis=1).mean()))
         for mode in (ADVANCE_NETWORK, ADVANCE_BALLISTIC):
             sur.set_macro(args.macro)
-            r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol)
+            r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol, options=opts_plain)
             err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean())
             res[mode].append({"wall": r["wall"], "err": err, "counts": r["counts"], "bad": r["nonfinite"]})
         # the same network with the other macro-step forms (which field moves the
         # particles, which advances F_E)
         for fl in variants:
             sur.set_macro(fl)
-            r = rollout(sur, Pq, ps, args.rollout, k, ADVANCE_NETWORK, tol=args.tol)
+            r = rollout(sur, Pq, ps, args.rollout, k, ADVANCE_NETWORK, tol=args.tol, options=opts_plain)
             var_res[fl].append(float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean()))
         sur.set_macro(args.macro)
         log(f"[learned] scene {i}: plain {ref['wall'] * 1e3:.1f} ms; network {res[ADVANCE_NETWORK][-1]['wall'] * 1e3:.1f} ms "
@@ -237,10 +261,15 @@ def learned(args) -> dict:
                         f"4-level BF16 U-Net 32-64-128-128 predicting grid velocity"
                         f"{' and displacement' if args.macro else ''}, target refinement {args.refine}, "
                         f"k=8 substeps per macro step, residual tolerance {args.tol}",
-            "train_scenes": ntr, "generated_substeps_per_scene": args.gen_substeps,
-            "train_samples": len(tr_slots), "train_steps": args.train_steps, "batch": args.batch,
+            "train_trajectories": trajectories, "stream_chunks": args.stream_chunks,
+            "stream_epochs": args.stream_epochs, "lr": args.lr,
+            "generated_substeps_per_scene": args.gen_substeps,
+            "train_samples": n_train, "train_steps": train_steps, "batch": args.batch,
+            "plain_options": {"sort_every": opts_plain.sort_every, "tile_particles": opts_plain.tile_particles,
+                              "hipgraph": True},
             "generate_s": round(gen_s, 2), "train_s": round(train_s, 2),
             "heldout_loss": round(val_loss, 5), "heldout_loss_ballistic": round(base_loss, 5),
+            "validation_loss_per_chunk": [round(v, 5) for v in curve],
             "test_scenes": nte, "rollout_substeps": args.rollout,
             "plain_ms": round(t_plain * 1e3, 3), "learned_ms": round(t_net * 1e3, 3),
             "mean_pos_err_m": err_net, "mean_pos_err_pct_domain": round(100 * err_net, 4),
@@ -272,6 +301,12 @@ def main() -> int:
     ap.add_argument("--test-scenes", type=int, default=4)
     ap.add_argument("--gen-substeps", type=int, default=1600)
     ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--stream-chunks", type=int, default=0,
+                    help="learned: train on this many chunks of --train-scenes new trajectories each, generated "
+                         "one chunk at a time (--train-steps spread over them); 0: one stored set")
+    ap.add_argument("--stream-epochs", type=int, default=1, help="learned: passes over the streamed chunks")
+    ap.add_argument("--patience", type=int, default=0,
+                    help="learned, streamed: stop after this many chunks without a lower held-out loss (0: never)")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--rollout", type=int, default=2000)

@@ -130,6 +130,13 @@ class Surrogate:
         keys = ("particle_mass", "grid_mass", "band_mass", "T", "gravity", "m_ref")
         return data, dict(zip(keys, map(float, sc)))
 
+    def get_scalars(self, slot: int) -> dict:
+        """The scalars of a slot only (no copy of its images)."""
+        sc = np.empty(6, np.float32)
+        check(self._lib.mpm_surrogate_get_sample(self._h, int(slot), None, _ptr(sc)))
+        keys = ("particle_mass", "grid_mass", "band_mass", "T", "gravity", "m_ref")
+        return dict(zip(keys, map(float, sc)))
+
     def train(self, slots, lr: float) -> float:
         s = np.ascontiguousarray(slots, np.int32)
         loss = np.zeros(1, np.float32)
@@ -193,7 +200,7 @@ def generate(sur: Surrogate, scenes, substeps: int, k: int = 8, first_slot: int 
             for i in range(len(scenes)):
                 sur.record(b, i, slot + i, 1, k)
                 if wall_tol is not None:
-                    _, sc = sur.get_sample(slot + i)
+                    sc = sur.get_scalars(slot + i)
                     pm = max(sc["particle_mass"], 1e-30)
                     if sc["band_mass"] / pm > wall_tol or abs(pm - sc["grid_mass"]) / pm > wall_tol:
                         continue
@@ -220,6 +227,71 @@ def fit(sur: Surrogate, train_slots, steps: int, batch: int = 8, lr: float = 1e-
         else:
             sur.train(sel, lr_t)
     return losses
+
+
+def train_streaming(sur: Surrogate, scene_fn, chunks: int, substeps: int, k: int = 8, first_slot: int = 0,
+                    steps_per_chunk: int = 1000, batch: int = 8, lr: float = 1e-3, lr_final: float | None = None,
+                    wall_tol: float | None = 1e-3, seed: int = 0, options=None, eval_fn=None, epochs: int = 1,
+                    patience: int = 0, log=print) -> dict:
+    """Training on more trajectories than the device holds at once: chunk c takes
+    the scenes scene_fn(c) (a list of (Params, ParticleSet)), runs them for
+    `substeps` substeps while recording one sample per scene every k substeps into
+    the slots from `first_slot` on (overwriting the previous chunk's), then takes
+    `steps_per_chunk` Adam steps on random batches of those samples; `epochs` passes
+    over the chunks (each pass regenerates them).  The learning rate decays on one
+    cosine over all steps.  With eval_fn (a function of the surrogate returning a
+    held-out loss) the loss is taken after every chunk, the parameters of the lowest
+    are kept and restored at the end, and with patience > 0 training stops once
+    that many chunks in a row have not lowered it by 0.5%.  Returns trajectory,
+    sample and step counts, the generation / training times and the losses."""
+    rng = np.random.default_rng(seed)
+    lr_final = lr * 0.1 if lr_final is None else lr_final
+    total = max(epochs * chunks * steps_per_chunk - 1, 1)
+    it, samples, gen_s, train_s, trajectories = 0, 0, 0.0, 0.0, 0
+    evals, best, best_p, stale = [], math.inf, None, 0
+    for e in range(epochs * chunks):
+        c = e % chunks
+        scenes = scene_fn(c)
+        t0 = time.perf_counter()
+        slots = generate(sur, scenes, substeps, k, first_slot=first_slot, options=options, wall_tol=wall_tol)
+        gen_s += time.perf_counter() - t0
+        if e < chunks:
+            trajectories += len(scenes)
+        pool = np.asarray([s for ss in slots for s in ss], np.int32)
+        samples += int(pool.size)
+        if pool.size == 0:
+            continue
+        t0 = time.perf_counter()
+        last, lr_t = 0.0, lr
+        for j in range(steps_per_chunk):
+            lr_t = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / total))
+            sel = rng.choice(pool, size=min(batch, pool.size), replace=False)
+            if j == steps_per_chunk - 1:
+                last = sur.train(sel, lr_t)
+            else:
+                sur.train(sel, lr_t)
+            it += 1
+        sur.loss(pool[:1])                      # waits for the chunk's last step
+        train_s += time.perf_counter() - t0
+        ev = ""
+        if eval_fn is not None:
+            evals.append(float(eval_fn(sur)))
+            ev = f", held-out {evals[-1]:.5f}"
+            if evals[-1] < best * 0.995:
+                stale = 0
+            else:
+                stale += 1
+            if evals[-1] < best:
+                best, best_p = evals[-1], sur.get_params()
+        log(f"[stream] chunk {e + 1}/{epochs * chunks}: {len(scenes)} trajectories, {pool.size} samples, "
+            f"{steps_per_chunk} steps, last batch loss {last:.4f}, lr {lr_t:.2e}{ev}")
+        if patience and stale >= patience:
+            log(f"[stream] held-out loss has not fallen for {patience} chunks: stopping")
+            break
+    if best_p is not None:
+        sur.set_params(best_p)
+    return {"trajectories": trajectories, "samples": samples, "steps": it, "generate_s": gen_s, "train_s": train_s,
+            "eval": evals, "best": best if best_p is not None else None}
 
 
 def rollout(sur: Surrogate | None, params, ps, substeps: int, k: int = 8, mode: int = ADVANCE_NETWORK,
