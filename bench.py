@@ -328,6 +328,9 @@ def main() -> int:
     ap.add_argument("--macro-variants", default="",
                     help="comma-separated further macro flags to roll the held-out scenes out with")
     ap.add_argument("--tile", type=int, default=512)
+    ap.add_argument("--fused", type=int, default=1,
+                    help="with --no-autotune: 1 per-block work items, 2 / 3 super-block work items (3D) with a "
+                         "drift margin of 0 / 1 cells")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=1)
@@ -403,12 +406,12 @@ def main() -> int:
         n = s.n
     else:
         if args.no_autotune:
-            opts = Options(sort_every=args.sort_every, fused=1, tile_particles=args.tile)
+            opts = Options(sort_every=args.sort_every, fused=args.fused, tile_particles=args.tile)
         else:
             # binning period and tile size from the autotuner on this scene (short
             # trials before the run, not timed); results do not depend on either
             opts, tuned_ms = autotune([(P, ps)], trial_substeps=64, graph=graph)
-            tuned = {"tile_particles": opts.tile_particles, "sort_every": opts.sort_every,
+            tuned = {"tile_particles": opts.tile_particles, "sort_every": opts.sort_every, "fused": opts.fused,
                      "ms_per_substep": round(tuned_ms, 5)}
             log(f"[bench] autotuned {tuned}")
         s = GPUSolver(P, ps, opts)
@@ -463,9 +466,13 @@ def main() -> int:
     value = n * args.steps / wall / 1e6
     avg_k = ms_k / max(nk, 1) / 1e3
     achieved = n * BYTES_PER_PARTICLE[dim] / avg_k / 1e9 if nk else None
-    kname = f"k_tiled<{dim},2,sticky,G2P{',batch' if wl == 'batch' else ''}> (fused G2P+P2G)"
+    if opts.fused >= 2:
+        kname = f"k_super<2,sticky,G2P,margin {opts.fused - 2}> (fused G2P+P2G, 2x2x2-block work items)"
+    else:
+        kname = f"k_tiled<{dim},2,sticky,G2P{',batch' if wl == 'batch' else ''}> (fused G2P+P2G)"
     config = {"workload": desc, "particles": n, "grid": grid, "sort_every": opts.sort_every,
-              "tile_particles": opts.tile_particles, "hipgraph": graph, "nonfinite_particles": bad}
+              "tile_particles": opts.tile_particles, "work_items": "super-block" if opts.fused >= 2 else "block",
+              "hipgraph": graph, "nonfinite_particles": bad}
     if tuned:
         config["autotuned"] = tuned
     out = {

@@ -18,7 +18,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libmpm_hip.so")
-SOURCES = ["mpm_kernels.hip", "mpm_fused.hip", "mpm_fused_cubic.hip", "mpm_sort.hip", "mpm_capi.hip",
+SOURCES = ["mpm_kernels.hip", "mpm_fused.hip", "mpm_fused_cubic.hip", "mpm_super.hip", "mpm_sort.hip", "mpm_capi.hip",
            "mpm_unet.hip", "mpm_surrogate_net.hip", "mpm_surrogate.hip", "mpm_io.hip", "mpm_noalloc.hip"]
 # per-source flags: the 3D cubic Coulomb-wall fused kernel unrolls its 64-node
 # stencil loops completely only above the default full-unroll limit

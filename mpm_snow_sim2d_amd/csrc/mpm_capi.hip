@@ -17,6 +17,7 @@
 #include "mpm_kernels.hpp"
 #include "mpm_solver_impl.hpp"
 #include "mpm_sort.hpp"
+#include "mpm_super.hpp"
 
 using namespace mpm;
 using namespace mpm::detail;
@@ -164,15 +165,17 @@ int reset_all_grids(mpm_solver* s)
     return MPM_OK;
 }
 
+SuperWork super_work(const mpm_solver* s);
+
 int do_sort(mpm_solver* s)
 {
     const int n = s->n;
     auto body = [&]() -> hipError_t {
         DevParticles p = s->dev_particles();
-        hipError_t e = s->scenes_d
-                           ? launch_cell_keys_batch(s->dim, p.x, n, s->scenes_d, s->p_off_d, s->nscenes, s->keys,
-                                                    s->stream)
-                           : launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
+        hipError_t e = s->scenes_d      ? launch_cell_keys_batch(s->dim, p.x, n, s->scenes_d, s->p_off_d, s->nscenes,
+                                                                 s->keys, s->stream)
+                       : s->super_items ? launch_cell_keys_super(p.x, n, s->ph, s->nsx, s->nsy, s->keys, s->stream)
+                                        : launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
         if (e != hipSuccess) return e;
         bool in_alt = false;
         e = radix_sort_pairs(s->keys, s->vals, s->keys_alt, s->vals_alt, n, s->key_bits, true,
@@ -197,14 +200,29 @@ int do_sort(mpm_solver* s)
         // the pair of key/value buffers the sort did not end in is free now
         uint32_t* tmp_a = in_alt ? s->keys : s->keys_alt;
         uint32_t* tmp_b = in_alt ? s->vals : s->vals_alt;
-        return launch_emit_items(skeys, n, s->opts.tile_particles, s->items, s->n_items, s->max_items, tmp_a,
-                                 tmp_b, s->radix_scratch, s->block_scene_d, s->stream);
+        e = launch_emit_items(skeys, n, s->opts.tile_particles, s->super_items ? SB_KEY_SHIFT : 6, s->items,
+                              s->n_items, s->max_items, tmp_a, tmp_b, s->radix_scratch, s->block_scene_d, s->stream);
+        if (e != hipSuccess || !s->super_items) return e;
+        // the new items' fixed-point bounds from the reordered state
+        return launch_item_bounds(s->cfg.order, s->dev_particles(), super_work(s), s->ph, s->stream);
     };
     int r = timed(s, MPM_KERNEL_SORT, body);
     if (r) return r;
     s->items_valid = true;
     s->since_sort = 0;
     return MPM_OK;
+}
+
+SuperWork super_work(const mpm_solver* s)
+{
+    SuperWork w;
+    w.items = s->items;
+    w.n_items = s->n_items;
+    w.max_items = s->max_items;
+    w.bound = s->item_bound;
+    w.nsx = s->nsx;
+    w.nsy = s->nsy;
+    return w;
 }
 
 TileWork tile_work(const mpm_solver* s)
@@ -229,9 +247,16 @@ int fused_substep(mpm_solver* s)
     });
     if (r) return r;
     DevParticles p = s->dev_particles();
-    TileWork w = tile_work(s);
-    r = timed(s, MPM_KERNEL_G2P2G,
-              [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->scenes_d, s->stream); });
+    if (s->super_items) {
+        const SuperWork w = super_work(s);
+        r = timed(s, MPM_KERNEL_G2P2G, [&] {
+            return launch_super(s->cfg, s->super_margin, true, p, cur, old, w, s->ph, s->stream);
+        });
+    } else {
+        TileWork w = tile_work(s);
+        r = timed(s, MPM_KERNEL_G2P2G,
+                  [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->scenes_d, s->stream); });
+    }
     if (r) return r;
     s->cur_grid = 1 - s->cur_grid;
     s->since_sort += 1;
@@ -285,9 +310,18 @@ int step_fused(mpm_solver* s, int nsub)
     if (s->grid_state != GRID_READY) {
         DevParticles p = s->dev_particles();
         DevGrid g = s->dev_grid(s->cur_grid);
-        TileWork w = tile_work(s);
-        r = timed(s, MPM_KERNEL_P2G,
-                  [&] { return launch_p2g_tiled(s->cfg, p, g, w, s->ph, s->scenes_d, s->stream); });
+        if (s->super_items) {
+            // bounds of the state as it is now (stage calls may have changed it)
+            const SuperWork w = super_work(s);
+            r = timed(s, MPM_KERNEL_P2G, [&] {
+                hipError_t e = launch_item_bounds(s->cfg.order, p, w, s->ph, s->stream);
+                return e != hipSuccess ? e : launch_super(s->cfg, s->super_margin, false, p, g, g, w, s->ph, s->stream);
+            });
+        } else {
+            TileWork w = tile_work(s);
+            r = timed(s, MPM_KERNEL_P2G,
+                      [&] { return launch_p2g_tiled(s->cfg, p, g, w, s->ph, s->scenes_d, s->stream); });
+        }
         if (r) return r;
         s->grid_state = GRID_READY;
     }
@@ -424,6 +458,9 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
     if (nscenes > 1 && params[0].scheme != MPM_SCHEME_MLS)
         return fail(MPM_E_INVALID, "a batch runs the MLS scheme");
     if (ntot > (int64_t)INT32_MAX / 16) return fail(MPM_E_INVALID, "particle count out of range");
+    if (opts && (opts->fused < 0 || opts->fused > 3)) return fail(MPM_E_INVALID, "fused must be 0, 1, 2 or 3");
+    if (opts && opts->fused >= 2 && (params[0].dim != 3 || nscenes != 1))
+        return fail(MPM_E_INVALID, "fused 2 and 3 (super-block work items) need one 3D scene");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1)
         return fail(MPM_E_HIP, "no HIP device available");
@@ -431,8 +468,13 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
     mpm_solver* s = new mpm_solver();
     s->params = params[0];
     if (opts) s->opts = *opts; else mpm_options_default(&s->opts);
-    // the tiled kernels hold at most two particles per thread of a 256-thread group
-    if (s->opts.tile_particles <= 0 || s->opts.tile_particles > 512) s->opts.tile_particles = 512;
+    // fused 2 / 3: super-block work items with a drift margin of 0 / 1 cells, for one
+    // 3D scene; the per-block tiled kernels hold at most two particles per thread of a
+    // 256-thread group, the super-block kernel streams up to SB_MAX_PARTICLES
+    s->super_items = s->opts.fused >= 2;
+    s->super_margin = s->opts.fused == 3 ? 1 : 0;
+    const int tile_max = s->super_items ? SB_MAX_PARTICLES : 512;
+    if (s->opts.tile_particles <= 0 || s->opts.tile_particles > tile_max) s->opts.tile_particles = tile_max;
     if (s->opts.sort_every < 0) s->opts.sort_every = 0;
     s->dim = params[0].dim;
     s->n = (int)ntot;
@@ -511,13 +553,25 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
     // binning buffers
     const size_t nn = (size_t)std::max(s->n, 1);
     uint64_t maxkey = (uint64_t)s->nblocks * 64;
+    int64_t nsuper = 0;
+    if (s->super_items) {
+        s->nsx = super_count(s->nbx);
+        s->nsy = super_count(s->nby);
+        s->nsz = super_count(s->nbz);
+        nsuper = (int64_t)s->nsx * s->nsy * s->nsz;
+        maxkey = (uint64_t)nsuper << SB_KEY_SHIFT;
+        if (maxkey > ((uint64_t)1 << 32)) return cleanup_fail(fail(MPM_E_INVALID, "grid too large for 32-bit super-block keys"));
+    }
     s->key_bits = 1;
     while (s->key_bits < 32 && ((uint64_t)1 << s->key_bits) < maxkey) ++s->key_bits;
     if (hipMalloc(&s->keys, nn * 4) != hipSuccess || hipMalloc(&s->vals, nn * 4) != hipSuccess ||
         hipMalloc(&s->keys_alt, nn * 4) != hipSuccess || hipMalloc(&s->vals_alt, nn * 4) != hipSuccess ||
         hipMalloc(&s->radix_scratch, radix_scratch_words(s->n) * 4) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "binning allocation failed"));
-    s->max_items = std::min(s->n, s->nblocks) + s->n / s->opts.tile_particles + 1;
+    s->max_items = (int)std::min<int64_t>(s->n, s->super_items ? nsuper : s->nblocks) + s->n / s->opts.tile_particles + 1;
+    if (s->super_items && (hipMalloc(&s->item_bound, (size_t)s->max_items * sizeof(float2)) != hipSuccess ||
+                           hipMemsetAsync(s->item_bound, 0, (size_t)s->max_items * sizeof(float2), s->stream) != hipSuccess))
+        return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
     if (hipMalloc(&s->items, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
         hipMalloc(&s->n_items, sizeof(int)) != hipSuccess || hipMalloc(&s->d_bad, sizeof(unsigned long long)) != hipSuccess)
         return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
@@ -638,19 +692,23 @@ int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* ini
     if (trial_substeps < 1) return fail(MPM_E_INVALID, "trial_substeps must be >= 1");
     static const int tiles[] = {128, 256, 512};
     static const int periods[] = {4, 8, 16, 32, 64};
+    // candidates (fused, tile_particles, sort_every): the per-block work items, and
+    // for one 3D scene the super-block items with either drift margin
+    std::vector<mpm_options> cands;
+    for (int tp : tiles)
+        for (int se : periods) cands.push_back(mpm_options{se, 1, tp});
+    if (nscenes == 1 && params && params[0].dim == 3)
+        for (int f : {2, 3})
+            for (int se : {8, 16, 32, 64}) cands.push_back(mpm_options{se, f, SB_MAX_PARTICLES});
     double best_ms = -1.0;
     mpm_options chosen{};
-    for (int tp : tiles)
-        for (int se : periods) {
-            mpm_options o{};
-            o.sort_every = se;
-            o.fused = 1;
-            o.tile_particles = tp;
+    for (const mpm_options& o : cands) {
+        {
             mpm_solver* s = nullptr;
             int r = create_impl(nscenes, params, init, &o, &s);
             if (r) return r;
             // whole binning periods, so that every candidate pays its binning share
-            const int period = 2 * se;
+            const int period = 2 * o.sort_every;
             const int trial = ((trial_substeps + period - 1) / period) * period;
             hipEvent_t a = nullptr, b = nullptr;
             float ms = 0.f;
@@ -673,6 +731,7 @@ int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* ini
                 chosen = o;
             }
         }
+    }
     *best = chosen;
     if (best_ms_per_substep) *best_ms_per_substep = best_ms;
     return MPM_OK;
@@ -694,6 +753,7 @@ int mpm_destroy(mpm_solver* s)
     }
     (void)hipFree(s->keys); (void)hipFree(s->vals); (void)hipFree(s->keys_alt); (void)hipFree(s->vals_alt);
     (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
+    (void)hipFree(s->item_bound);
     (void)hipFree(s->scenes_d); (void)hipFree(s->block_scene_d); (void)hipFree(s->p_off_d);
     drop_graph(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -66,6 +66,12 @@ struct mpm_solver {
     int4* items = nullptr;
     int* n_items = nullptr;
     int max_items = 0;
+    // super-block work items (mpm_options.fused 2 or 3, 3D single scene): keys and
+    // items per 2x2x2 grid blocks, and each item's bound for its fixed-point scale
+    bool super_items = false;
+    int super_margin = 0;
+    int nsx = 0, nsy = 0, nsz = 0;
+    float2* item_bound = nullptr;
     bool items_valid = false;
     bool sorted_keys_in_alt = false;
     int since_sort = 0;

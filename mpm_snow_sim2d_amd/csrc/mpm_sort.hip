@@ -269,20 +269,21 @@ __global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__
     if (s < n) dst[s] = src[perm[s]];
 }
 
-// Work items: every run of equal block id (key >> 6) in the sorted keys, split into
-// chunks of at most tile_p particles; item.w is the block's scene (0 for one
-// scene).  Deterministic: a run start writes its run length and chunk count, an
+// Work items: every run of equal block id (key >> shift: 6 for grid blocks, 9 for
+// the super-blocks of mpm_super.hip) in the sorted keys, split into the fewest
+// chunks of at most tile_p particles, of equal size (within one particle); item.w
+// is the block's scene (0 for one scene).  Deterministic: a run start writes its run length and chunk count, an
 // exclusive scan of the counts places the chunks, so the items come out in key
 // (spatial) order.
-__global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n, int tile_p,
+__global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n, int tile_p, int shift,
                                                     uint32_t* runlen, uint32_t* counts)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    uint32_t b = keys[s] >> 6;
-    if (s > 0 && (keys[s - 1] >> 6) == b) { runlen[s] = 0; counts[s] = 0; return; }
+    uint32_t b = keys[s] >> shift;
+    if (s > 0 && (keys[s - 1] >> shift) == b) { runlen[s] = 0; counts[s] = 0; return; }
     // end of the run: first index with block id > b (binary search, keys sorted)
-    uint32_t next = (b + 1u) << 6;
+    uint32_t next = (b + 1u) << shift;
     int lo = s + 1, hi = n;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -293,7 +294,7 @@ __global__ void __launch_bounds__(256) k_item_counts(const uint32_t* keys, int n
     counts[s] = (len + tile_p - 1) / tile_p;
 }
 
-__global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n, int tile_p,
+__global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n, int tile_p, int shift,
                                                    const uint32_t* runlen, const uint32_t* offsets,
                                                    int4* items, int* n_items, int max_items,
                                                    const int* block_scene)
@@ -305,10 +306,14 @@ __global__ void __launch_bounds__(256) k_item_write(const uint32_t* keys, int n,
     int at = (int)offsets[s];
     if (s == n - 1) *n_items = min(at + nch, max_items);
     if (len == 0) return;
-    int b = (int)(keys[s] >> 6);
+    int b = (int)(keys[s] >> shift);
     int scene = block_scene ? block_scene[b] : 0;
-    for (int c = 0; c < nch && at + c < max_items; ++c)
-        items[at + c] = make_int4(b, s + c * tile_p, min(tile_p, len - c * tile_p), scene);
+    // nch chunks of equal size (within one particle), each at most tile_p: chunk c
+    // holds run positions [c len / nch, (c + 1) len / nch)
+    for (int c = 0; c < nch && at + c < max_items; ++c) {
+        const int lo = (int)((long long)c * len / nch), hi = (int)((long long)(c + 1) * len / nch);
+        items[at + c] = make_int4(b, s + lo, hi - lo, scene);
+    }
 }
 
 __global__ void k_zero_int(int* v) { *v = 0; }
@@ -349,17 +354,17 @@ hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int* n_items, int max_items,
-                             uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
+hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int shift, int4* items, int* n_items,
+                             int max_items, uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
                              const int* block_scene, hipStream_t st)
 {
     k_zero_int<<<1, 1, 0, st>>>(n_items);
     if (n <= 0) return hipGetLastError();
     dim3 grid((n + 255) / 256), blk(256);
-    k_item_counts<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts);
+    k_item_counts<<<grid, blk, 0, st>>>(keys, n, tile_p, shift, tmp_runlen, tmp_counts);
     hipError_t e = exclusive_scan(tmp_counts, n, scan_sums, st);
     if (e != hipSuccess) return e;
-    k_item_write<<<grid, blk, 0, st>>>(keys, n, tile_p, tmp_runlen, tmp_counts, items, n_items, max_items,
+    k_item_write<<<grid, blk, 0, st>>>(keys, n, tile_p, shift, tmp_runlen, tmp_counts, items, n_items, max_items,
                                       block_scene);
     return hipGetLastError();
 }

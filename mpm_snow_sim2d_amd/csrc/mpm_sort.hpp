@@ -40,11 +40,11 @@ hipError_t launch_gather_fields(const float* src, float* dst, const uint32_t* pe
                                 hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t* perm, int n,
                              hipStream_t st);
-// Work items (block id, first particle, count, scene) from sorted keys, in key
-// order; tmp_runlen and tmp_counts hold n words each, scan_sums n/1024 + 1;
-// block_scene (batch-wide block -> scene) may be NULL for one scene.
-hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int4* items, int* n_items, int max_items,
-                             uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
+// Work items (block id = key >> shift, first particle, count, scene) from sorted
+// keys, in key order; tmp_runlen and tmp_counts hold n words each, scan_sums
+// n/1024 + 1; block_scene (batch-wide block -> scene) may be NULL for one scene.
+hipError_t launch_emit_items(const uint32_t* keys, int n, int tile_p, int shift, int4* items, int* n_items,
+                             int max_items, uint32_t* tmp_runlen, uint32_t* tmp_counts, uint32_t* scan_sums,
                              const int* block_scene, hipStream_t st);
 hipError_t launch_cell_keys_batch(int dim, const float* x, int n, const SceneDesc* scenes, const int* p_off,
                                   int nscenes, uint32_t* keys, hipStream_t st);

@@ -1,0 +1,626 @@
+// mpm_super.hip — the 3D fused substep over super-block work items (mpm_options
+// fused = 2 or 3): one workgroup takes the particles binned into 2x2x2 grid blocks
+// (8^3 cells) and runs G2P (shaders/3d/g2p.glsl:22-105) and the P2G of the updated
+// particle (shaders/3d/p2g.glsl:31-97) in ONE pass per particle, through an LDS tile
+// of the 8^3 cells plus the stencil reach and a drift margin of MG cells.
+//
+// Against the per-block kernel k_tiled (mpm_fused_kernel.hpp):
+//   * the tile is staged and flushed once per ~8x more particles (a 4^3 block's
+//     9^3-node tile carries ~2.6 staged and ~1.2 flushed nodes per particle at 4.4
+//     particles per cell; an 8^3 super-block's 11^3 tile 0.6 and 0.6);
+//   * no barrier between G2P and the scatter: k_tiled must finish the G2P of all
+//     its particles before it knows the item's fixed-point scale, and keeps their
+//     scatter inputs in registers across that barrier; here the scale comes from
+//     the bound the same particles had one substep earlier (written by this kernel
+//     at the end of each item, and after each binning by k_item_bounds from the
+//     stored state, which is exactly what the previous substep scattered), plus the
+//     momentum gravity adds in one substep, with a factor 2 of headroom.  A particle whose bound exceeds it this substep scatters
+//     with global float atomics instead, so results never depend on the headroom.
+//
+// Fixed-point accumulation as in k_tiled: per component an integer sum in LDS, two
+// components per 64-bit add.  With cnt particles in the item a node receives at
+// most cnt contributions, each |w * value| * 2^e <= 0.42 * 2^cap (w <= 0.75^3 for
+// the quadratic, (2/3)^3 for the cubic stencil), with cap = min(31 - ceil(log2 cnt), 21),
+// so each 32-bit half of a node's sum stays below 2^31, and every contribution is
+// below 2^22 as the rounding trick of fixed_bits needs.
+#include "mpm_fused_kernel.hpp"
+#include "mpm_super.hpp"
+
+namespace mpm {
+
+namespace {
+
+// |m|, the scatter's inputs m v and A = m C - (V D^-1 dt) tau, and the bound on the
+// magnitude of any one of its momentum contributions, |m v_r| + reach dx sum_c |A_rc|
+// (the same quantities k_tiled computes for its scale).
+template <int ORDER>
+__device__ inline float scatter_inputs(const Phys& ph, float pm, float vol, const float* v, const float* C,
+                                       const float* tau, float* mvp, float* A)
+{
+    constexpr float REACH = ORDER == 3 ? 2.f : 1.5f;
+    float dinv = ph.Dinv;
+    asm volatile("" : "+v"(dinv));           // formed per particle, not hoisted and kept live
+    const float vfd = vol * (dinv * ph.dt);
+    float bp = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        mvp[r] = pm * v[r];
+        float rowsum = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float a = pm * C[r * 3 + c] - vfd * tau[r * 3 + c];
+            A[r * 3 + c] = a;
+            rowsum += fabsf(a);
+        }
+        bp = fmaxf(bp, fabsf(mvp[r]) + REACH * ph.dx * rowsum);
+    }
+    return bp;
+}
+
+// 2^e with (2 * bound) * 2^e < 2^cap; a zero or non-finite bound gives 2^126, so
+// that every particle with a nonzero contribution takes the float path.
+__device__ inline float headroom_scale(float bound, int cap)
+{
+    if (!(bound > 0.f) || !(bound < 1.0e37f)) return ldexpf(1.f, 126);
+    int k;
+    frexpf(2.f * bound, &k);                 // 2 bound < 2^k
+    return ldexpf(1.f, min(max(cap - k, -126), 126));
+}
+
+}  // namespace
+
+// Largest |m| and largest momentum bound of each item's particles in the stored
+// state (shaders/3d/p2g.glsl:44-59 stress), i.e. what a scatter of that state
+// would need: written after every binning and before a P2G-only pass.
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork work, Phys ph)
+{
+    __shared__ float s_red[2][4];
+    const int n = P.n;
+    const int nit = *work.n_items;
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int4 item = work.items[it];
+        float bm = 0.f, bpm = 0.f;
+        for (int q = threadIdx.x; q < item.z; q += 256) {
+            const int p = item.y + q;
+            const float vol = P.vol[p];
+            if (vol == 0.f) continue;                                // p2g.glsl:42
+            const float pm = P.mass[p], Jp = P.Jp[p];
+            float v[3], C[9], F[9], U[9], sig[3], V[9], tau[9], mvp[3], A[9];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = P.v[d * n + p];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) { C[e] = P.C[e * n + p]; F[e] = P.F[e * n + p]; }
+            svd3(F, U, sig, V);
+            kirchhoff<3>(ph, F, U, V, det<3>(F), Jp, tau);
+            const float bp = scatter_inputs<ORDER>(ph, pm, vol, v, C, tau, mvp, A);
+            if (bp < 1.0e37f && fabsf(pm) < 1.0e37f) {
+                bm = fmaxf(bm, fabsf(pm));
+                bpm = fmaxf(bpm, bp);
+            }
+        }
+        bm = wave_max(bm);
+        bpm = wave_max(bpm);
+        if ((threadIdx.x & 63) == 0) { s_red[0][threadIdx.x >> 6] = bm; s_red[1][threadIdx.x >> 6] = bpm; }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            work.bound[it] = make_float2(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])),
+                                         fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
+        __syncthreads();
+    }
+}
+
+template <int ORDER, bool COULOMB, bool G2P, int MG>
+__global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, DevGrid dst, SuperWork work,
+                                                  Phys ph)
+{
+    constexpr int T = SB_CELLS + 3 + 2 * MG;          // tile edge (nodes)
+    constexpr int TN = T * T * T;
+    constexpr int NF = SB_BLOCKS + 2;                 // grid blocks a tile overlaps per axis
+    constexpr int W = Spline<ORDER>::W;
+    constexpr int NVR = COULOMB ? 6 : 3;
+    // accumulators: per tile node (mass, p_x) and (p_y, p_z) as packed fixed point
+    __shared__ unsigned long long s_acc[2][TN];
+    // gather velocities, component-major: friction velocity in rows 0..2, collision
+    // velocity in rows 3..5 with Coulomb walls
+    __shared__ float s_vel[G2P ? NVR : 1][G2P ? TN : 1];
+    __shared__ int s_touch[NF * NF * NF];
+    __shared__ float s_red[2][4];
+
+    const int n = P.n;
+    const int tid = threadIdx.x;
+    const int nit = *work.n_items;
+
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int4 item = work.items[it];
+        const int first = item.y, cnt = min(item.z, SB_MAX_PARTICLES);
+        int sc[3];
+        sc[0] = item.x % work.nsx;
+        const int rest = item.x / work.nsx;
+        sc[1] = rest % work.nsy;
+        sc[2] = rest / work.nsy;
+        int o[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o[a] = sc[a] * SB_CELLS - 1 - MG;
+
+        // ---- stage the tile (all loads issued before the first LDS store)
+        int ltid = tid;
+        asm volatile("" : "+v"(ltid));
+        {
+            constexpr int NR = (TN + 255) / 256;
+            float4 va[NR], vb[NR];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                va[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                vb[k] = va[k];
+                const int e = ltid + 256 * k;
+                if (G2P && e < TN) {
+                    const int i = o[0] + e % T, j = o[1] + (e / T) % T, kk = o[2] + e / (T * T);
+                    if (in_grid<3>(ph, i, j, kk)) {
+                        const int addr = node_addr<3>(src, i, j, kk);
+                        va[k] = src.a[addr];
+                        if constexpr (COULOMB) vb[k] = src.vc[addr];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const int e = ltid + 256 * k;
+                if (e >= TN) continue;
+                s_acc[0][e] = 0ull;
+                s_acc[1][e] = 0ull;
+                if constexpr (G2P) {
+                    s_vel[0][e] = va[k].y;
+                    s_vel[1][e] = va[k].z;
+                    s_vel[2][e] = va[k].w;
+                    if constexpr (COULOMB) {
+                        s_vel[3 % NVR][e] = vb[k].x;
+                        s_vel[4 % NVR][e] = vb[k].y;
+                        s_vel[5 % NVR][e] = vb[k].z;
+                    }
+                }
+            }
+        }
+        if (tid < NF * NF * NF) s_touch[tid] = 0;
+
+        // ---- the item's fixed-point scales, from its particles' bounds one substep
+        // earlier plus the momentum gravity adds in a substep (so that a body at
+        // rest has a scale for its first substep), with a factor 2 of headroom
+        const int lg = cnt > 1 ? 32 - __clz(cnt - 1) : 0;     // ceil(log2 cnt)
+        const int cap = min(31 - lg, 21);
+        const float lim = ldexpf(1.f, cap);
+        const float2 prev = work.bound[it];
+        const float scale_m = headroom_scale(prev.x, cap);
+        const float scale_p = headroom_scale(prev.y + prev.x * (fabsf(ph.gravity) * ph.dt), cap);
+        lds_barrier();
+
+        float bound_m = 0.f, bound_p = 0.f;
+        const int npass = (cnt + 63) >> 6;
+        for (int pass = ((tid >> 6) + it) & 3; pass < npass; pass += 4) {
+            const int q = pass * 64 + (tid & 63);
+            if (q >= cnt) continue;
+            const int p = first + q;
+            float x[3], v[3], C[9], tau[9], Jp;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) x[d] = P.x[d * n + p];
+            Jp = P.Jp[p];
+            const float vol = P.vol[p];
+            const float pm = P.mass[p];
+
+            if constexpr (G2P) {
+                // g2p.glsl: gather, B (C = D^-1 B), position, F_E, plasticity
+                float xg[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) xg[d] = x[d] * ph.inv_dx;
+                Stencil<3, ORDER> st;
+                st.init(xg);
+                bool tile_ok = true, inside;
+                int rb[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    rb[a] = st.base[a] - o[a];
+                    tile_ok &= rb[a] >= 0 && rb[a] + W <= T;
+                }
+                inside = tile_ok;
+                {
+                    const int lim3[3] = {ph.W, ph.H, ph.D};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) inside &= st.base[a] >= 0 && st.base[a] + W - 1 <= lim3[a];
+                }
+                float nv[3] = {0.f, 0.f, 0.f}, nB[9], nx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int e = 0; e < 9; ++e) nB[e] = 0.f;
+                if (inside) {
+                    // the node weight is a product of per-axis weights, so the sums
+                    // over the W^3 nodes factor axis by axis (as in k_tiled)
+                    float wd[3][W];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) wd[a][s2] = -st.w[a][s2] * (st.d[a][s2] * ph.dx);
+                    float nc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int a0 = 0; a0 < W; ++a0) {
+                        float S1[3] = {0.f, 0.f, 0.f}, D1[3] = {0.f, 0.f, 0.f}, D2[3] = {0.f, 0.f, 0.f},
+                              C1[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int a1 = 0; a1 < W; ++a1) {
+                            float S2[3] = {0.f, 0.f, 0.f}, E2[3] = {0.f, 0.f, 0.f}, C2[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int a2 = 0; a2 < W; ++a2) {
+                                const int e = ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0);
+                                const float w2 = st.w[2][a2], wd2 = wd[2][a2];
+#pragma unroll
+                                for (int d = 0; d < 3; ++d) {
+                                    const float vF = s_vel[d][e];
+                                    S2[d] += w2 * vF;
+                                    E2[d] += wd2 * vF;
+                                    if constexpr (COULOMB) C2[d] += w2 * s_vel[(3 + d) % NVR][e];
+                                }
+                            }
+                            const float w1 = st.w[1][a1], wd1 = wd[1][a1];
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) {
+                                S1[d] += w1 * S2[d];
+                                D1[d] += wd1 * S2[d];
+                                D2[d] += w1 * E2[d];
+                                if constexpr (COULOMB) C1[d] += w1 * C2[d];
+                            }
+                        }
+                        const float w0 = st.w[0][a0], wd0 = wd[0][a0];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            nv[d] += w0 * S1[d];
+                            nB[d * 3 + 0] += wd0 * S1[d];
+                            nB[d * 3 + 1] += w0 * D1[d];
+                            nB[d * 3 + 2] += w0 * D2[d];
+                            if constexpr (COULOMB) nc[d] += w0 * C1[d];
+                        }
+                    }
+                    float sw[3], mi[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        sw[a] = 0.f;
+                        mi[a] = 0.f;
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) {
+                            sw[a] += st.w[a][s2];
+                            mi[a] += st.w[a][s2] * (float)(st.base[a] + s2);
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        float m = mi[d];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            if (a != d) m *= sw[a];
+                        nx[d] = m * ph.dx + ph.dt * (COULOMB ? nc[d] : nv[d]);
+                    }
+                } else {
+                    // stencil reaching past the grid or the tile: per-node tests,
+                    // from the tile where it holds the node, else from HBM
+#pragma unroll
+                    for (int a0 = 0; a0 < W; ++a0)
+#pragma unroll
+                        for (int a1 = 0; a1 < W; ++a1)
+#pragma unroll
+                            for (int a2 = 0; a2 < W; ++a2) {
+                                const int idx[3] = {st.base[0] + a0, st.base[1] + a1, st.base[2] + a2};
+                                if (!in_grid<3>(ph, idx[0], idx[1], idx[2])) continue;
+                                const float w = st.w[0][a0] * st.w[1][a1] * st.w[2][a2];
+                                const float dp[3] = {st.d[0][a0] * ph.dx, st.d[1][a1] * ph.dx, st.d[2][a2] * ph.dx};
+                                float vF[3], vC[3];
+                                if (tile_ok) {
+                                    const int e = ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0);
+#pragma unroll
+                                    for (int d = 0; d < 3; ++d) {
+                                        vF[d] = s_vel[d][e];
+                                        vC[d] = COULOMB ? s_vel[(3 + d) % NVR][e] : vF[d];
+                                    }
+                                } else {
+                                    const int addr = node_addr<3>(src, idx[0], idx[1], idx[2]);
+                                    const float4 a4 = src.a[addr];
+                                    vF[0] = a4.y; vF[1] = a4.z; vF[2] = a4.w;
+                                    if constexpr (COULOMB) {
+                                        const float4 c4 = src.vc[addr];
+                                        vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
+                                    } else {
+                                        vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
+                                    }
+                                }
+#pragma unroll
+                                for (int r = 0; r < 3; ++r) {
+                                    nv[r] += vF[r] * w;
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) nB[r * 3 + c] -= vF[r] * (w * dp[c]);
+                                    nx[r] += (ph.dt * vC[r] + (float)idx[r] * ph.dx) * w;
+                                }
+                            }
+                }
+                float M[9], Fo[9], Fh[9], F[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    C[e] = ph.Dinv * nB[e];
+                    M[e] = ((e % 4) == 0 ? 1.f : 0.f) + ph.dt * C[e];
+                    Fo[e] = P.F[e * n + p];
+                }
+                matmul<3>(M, Fo, Fh);
+                float U[9], sig[3], V[9], scv[3];
+                svd3(Fh, U, sig, V);
+                float ratio = 1.f, J = 1.f;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    float sv = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
+                    sv = fmaxf(sv, 1e-6f);
+                    scv[d] = sv;
+                    ratio *= sig[d] / sv;
+                    J *= sv;
+                }
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        float sacc = (U[r * 3] * scv[0]) * V[c * 3];
+#pragma unroll
+                        for (int qq = 1; qq < 3; ++qq) sacc += (U[r * 3 + qq] * scv[qq]) * V[c * 3 + qq];
+                        F[r * 3 + c] = sacc;
+                    }
+                Jp = Jp * ratio;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { x[d] = nx[d]; v[d] = nv[d]; }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { P.x[d * n + p] = x[d]; P.v[d * n + p] = v[d]; }
+#pragma unroll
+                for (int e = 0; e < 9; ++e) { P.C[e * n + p] = C[e]; P.F[e * n + p] = F[e]; }
+                P.Jp[p] = Jp;
+                // stress of the new state: U diag(2 mu (s-1) s + la (J-1) J) U^T
+                const float harden = expf(ph.xi * (1.f - Jp));
+                const float tmu = 2.f * (ph.mu * harden);
+                const float sI = (ph.lambda * harden) * (J - 1.f) * J;
+                float tdiag[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) tdiag[d] = tmu * (scv[d] - 1.f) * scv[d] + sI;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        float sacc = U[r * 3] * tdiag[0] * U[c * 3];
+#pragma unroll
+                        for (int qq = 1; qq < 3; ++qq) sacc += U[r * 3 + qq] * tdiag[qq] * U[c * 3 + qq];
+                        tau[r * 3 + c] = sacc;
+                    }
+            } else {
+                // stress from the stored state (p2g.glsl:44-59)
+                float F[9], U[9], sig[3], V[9];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[d] = P.v[d * n + p];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) { C[e] = P.C[e * n + p]; F[e] = P.F[e * n + p]; }
+                svd3(F, U, sig, V);
+                kirchhoff<3>(ph, F, U, V, det<3>(F), Jp, tau);
+            }
+
+            if (vol == 0.f) continue;                                 // p2g.glsl:42
+            // ---- scatter (p2g.glsl): into the tile when the stencil lies in it and
+            // the contributions fit the item's scale, else global float atomics
+            float mvp[3], A[9];
+            const float bp = scatter_inputs<ORDER>(ph, pm, vol, v, C, tau, mvp, A);
+            const bool finite = bp < 1.0e37f && fabsf(pm) < 1.0e37f;
+            if (finite) {
+                bound_m = fmaxf(bound_m, fabsf(pm));
+                bound_p = fmaxf(bound_p, bp);
+            }
+            float xg[3];
+            int base[3], rb[3];
+            bool tile_ok = finite && bp * scale_p <= lim && fabsf(pm) * scale_m <= lim;
+            bool inside = true;
+            {
+                const int lim3[3] = {ph.W, ph.H, ph.D};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    xg[d] = x[d] * ph.inv_dx;
+                    base[d] = Spline<ORDER>::base(xg[d]);
+                    rb[d] = base[d] - o[d];
+                    tile_ok &= rb[d] >= 0 && rb[d] + W <= T;
+                    inside &= base[d] >= 0 && base[d] + W - 1 <= lim3[d];
+                }
+            }
+            float wa[3][W], xa[3][W];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int s2 = 0; s2 < W; ++s2) {
+                    const float dd = xg[d] - (float)(base[d] + s2);
+                    wa[d][s2] = Spline<ORDER>::N(dd);
+                    xa[d][s2] = -dd * ph.dx;
+                }
+            if (tile_ok && inside) {
+                // whole stencil in the tile and the grid: scaled inputs, one rounding
+                // per component, two packed 64-bit LDS adds per node; each lane walks
+                // its stencil from a start rotated by its lane index, so that sorted
+                // neighbours with one stencil base add to different words at a time
+                const float pms = pm * scale_m;
+                float mvs[3], As[9];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    mvs[r] = mvp[r] * scale_p;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) As[r * 3 + c] = A[r * 3 + c] * scale_p;
+                }
+                float wr[3][W], xr[3][W];
+                int er[3][W];
+                {
+                    const int strd[3] = {1, T, T * T};
+                    int lr = tid & 63;
+                    asm volatile("" : "+v"(lr));
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int rot = lr % W;
+                        lr /= W;
+#pragma unroll
+                        for (int s2 = 0; s2 < W; ++s2) {
+                            const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
+                            float wv = wa[d][0], xv = xa[d][0];
+#pragma unroll
+                            for (int t2 = 1; t2 < W; ++t2)
+                                if (idx == t2) { wv = wa[d][t2]; xv = xa[d][t2]; }
+                            wr[d][s2] = wv;
+                            xr[d][s2] = xv;
+                            er[d][s2] = (rb[d] + idx) * strd[d];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int a0 = 0; a0 < W; ++a0)
+#pragma unroll
+                    for (int a1 = 0; a1 < W; ++a1) {
+                        const float w01 = wr[0][a0] * wr[1][a1];
+                        float sb[3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+                            sb[r] = (mvs[r] + As[r * 3] * xr[0][a0]) + As[r * 3 + 1] * xr[1][a1];
+#pragma unroll
+                        for (int a2 = 0; a2 < W; ++a2) {
+                            const float w = w01 * wr[2][a2];
+                            int qb[4];
+                            qb[0] = fixed_bits(w, pms);
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) qb[1 + r] = fixed_bits(w, sb[r] + As[r * 3 + 2] * xr[2][a2]);
+                            const int e = er[0][a0] + er[1][a1] + er[2][a2];
+                            atomicAdd(&s_acc[0][e], pack_fixed(qb[0], qb[1]));
+                            atomicAdd(&s_acc[1][e], pack_fixed(qb[2], qb[3]));
+                        }
+                    }
+            } else {
+#pragma unroll
+                for (int a0 = 0; a0 < W; ++a0)
+#pragma unroll
+                    for (int a1 = 0; a1 < W; ++a1)
+#pragma unroll
+                        for (int a2 = 0; a2 < W; ++a2) {
+                            const int idx[3] = {base[0] + a0, base[1] + a1, base[2] + a2};
+                            if (!in_grid<3>(ph, idx[0], idx[1], idx[2])) continue;
+                            const float w = wa[0][a0] * wa[1][a1] * wa[2][a2];
+                            const float xi[3] = {xa[0][a0], xa[1][a1], xa[2][a2]};
+                            float vals[4];
+                            node_contrib<3>(w, pm, mvp, A, xi, vals);
+                            if (tile_ok) {
+                                const int e = ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0);
+                                const int q0 = fixed_bits(vals[0], scale_m), q1 = fixed_bits(vals[1], scale_p);
+                                const int q2 = fixed_bits(vals[2], scale_p), q3 = fixed_bits(vals[3], scale_p);
+                                atomicAdd(&s_acc[0][e], pack_fixed(q0, q1));
+                                atomicAdd(&s_acc[1][e], pack_fixed(q2, q3));
+                            } else {
+                                float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, idx[0], idx[1], idx[2])]);
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) unsafeAtomicAdd(dstp + c, vals[c]);
+                                dst.flags[block_of<3>(dst, idx[0], idx[1], idx[2])] = 1u;
+                            }
+                        }
+            }
+        }
+
+        // ---- this substep's bounds, for the next substep's scales
+        bound_m = wave_max(bound_m);
+        bound_p = wave_max(bound_p);
+        if ((tid & 63) == 0) { s_red[0][tid >> 6] = bound_m; s_red[1][tid >> 6] = bound_p; }
+        lds_barrier();
+        if (tid == 0)
+            work.bound[it] = make_float2(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])),
+                                         fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
+
+        // ---- flush: fixed point back to FP32, one global atomic per touched node
+        // component, lanes over (node, component) pairs with the component fastest
+        const float inv_m = 1.f / scale_m, inv_p = 1.f / scale_p;
+        bool tile_in = true;
+        {
+            const int lim3[3] = {ph.W, ph.H, ph.D};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) tile_in &= o[a] >= 0 && o[a] + T - 1 <= lim3[a];
+        }
+        for (int pe = tid; pe < TN * 4; pe += 256) {
+            const int e = pe >> 2, c = pe & 3;
+            const unsigned long long sum = s_acc[c >> 1][e];
+            const int acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
+            if (acc == 0) continue;
+            const int i = o[0] + e % T, j = o[1] + (e / T) % T, k = o[2] + e / (T * T);
+            if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
+            float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, i, j, k)]) + c;
+            unsafeAtomicAdd(dstp, (float)acc * (c == 0 ? inv_m : inv_p));
+            const int rbx = (i >> 2) - (sc[0] * SB_BLOCKS - 1);
+            const int rby = (j >> 2) - (sc[1] * SB_BLOCKS - 1);
+            const int rbz = (k >> 2) - (sc[2] * SB_BLOCKS - 1);
+            s_touch[(rbz * NF + rby) * NF + rbx] = 1;
+        }
+        lds_barrier();
+        int ftid = tid;
+        asm volatile("" : "+v"(ftid));
+        if (ftid < NF * NF * NF && s_touch[ftid]) {
+            const int bx = sc[0] * SB_BLOCKS - 1 + ftid % NF;
+            const int by = sc[1] * SB_BLOCKS - 1 + (ftid / NF) % NF;
+            const int bz = sc[2] * SB_BLOCKS - 1 + ftid / (NF * NF);
+            dst.flags[(bz * dst.nby + by) * dst.nbx + bx] = 1u;
+        }
+        lds_barrier();
+    }
+}
+
+// Super-block binning keys: the cell containing the particle, clamped into the
+// grid, numbered super-block-major: key = ((sz * nsy + sy) * nsx + sx) * 512 +
+// (block within the super-block) * 64 + (cell within the block, x fastest).
+__global__ void __launch_bounds__(256) k_cell_keys_super(const float* x, int n, float inv_dx, int W, int H, int D,
+                                                         int nsx, int nsy, uint32_t* keys)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int lim[3] = {W, H, D};
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = min(max(cell_floor(x[d * n + p] * inv_dx), 0), lim[d]);
+    keys[p] = super_key(c[0], c[1], c[2], nsx, nsy);
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, uint32_t* keys,
+                                  hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    k_cell_keys_super<<<(n + 255) / 256, 256, 0, st>>>(x, n, ph.inv_dx, ph.W, ph.H, ph.D, nsx, nsy, keys);
+    return hipGetLastError();
+}
+
+static int super_grid(const SuperWork& w)
+{
+    int g = w.max_items < 4096 ? w.max_items : 4096;
+    g = (g + 7) & ~7;
+    return g > 0 ? g : 8;
+}
+
+hipError_t launch_item_bounds(int order, const DevParticles& p, const SuperWork& w, const Phys& ph,
+                              hipStream_t st)
+{
+    if (p.n == 0) return hipSuccess;
+    if (order == 3) k_item_bounds<3><<<super_grid(w), 256, 0, st>>>(p, w, ph);
+    else            k_item_bounds<2><<<super_grid(w), 256, 0, st>>>(p, w, ph);
+    return hipGetLastError();
+}
+
+hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, const DevParticles& p, const DevGrid& src,
+                        const DevGrid& dst, const SuperWork& w, const Phys& ph, hipStream_t st)
+{
+    if (p.n == 0) return hipSuccess;
+    if (c.dim != 3) return hipErrorInvalidValue;
+    const dim3 grid(super_grid(w)), blk(256);
+    // the P2G-only pass gathers nothing, so the wall kind does not enter it
+    const bool coul = g2p && c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
+#define SCASE(O, CL, G, M) \
+    if (c.order == O && coul == CL && g2p == G && margin == M) { k_super<O, CL, G, M><<<grid, blk, 0, st>>>(p, src, dst, w, ph); return hipGetLastError(); }
+    SCASE(2, false, true, 0) SCASE(2, false, true, 1) SCASE(2, true, true, 0) SCASE(2, true, true, 1)
+    SCASE(2, false, false, 0) SCASE(2, false, false, 1)
+    SCASE(3, false, true, 0) SCASE(3, false, true, 1) SCASE(3, true, true, 0) SCASE(3, true, true, 1)
+    SCASE(3, false, false, 0) SCASE(3, false, false, 1)
+#undef SCASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mpm

@@ -1,0 +1,48 @@
+// mpm_super.hpp — super-block work items of the 3D fused substep (mpm_super.hip):
+// binning keys, per-item bounds and the launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpm_kernels.hpp"
+
+namespace mpm {
+
+constexpr int SB_BLOCKS = 2;                 // grid blocks per axis of a super-block
+constexpr int SB_CELLS = 4 * SB_BLOCKS;      // cells per axis of a super-block
+constexpr int SB_KEY_SHIFT = 9;              // key bits below the super-block id: 8 blocks x 64 cells
+constexpr int SB_MAX_PARTICLES = 2048;       // particles per work item at most
+
+struct SuperWork {
+    const int4* items;   // (super-block id, first particle, count, 0) per work item
+    const int*  n_items;
+    int         max_items;
+    float2*     bound;   // per item: largest |m| and largest momentum bound of its particles
+    int         nsx, nsy;   // super-blocks along x and y
+};
+
+// Super-blocks per axis for nb grid blocks along it.
+__host__ __device__ constexpr int super_count(int nb) { return (nb + SB_BLOCKS - 1) / SB_BLOCKS; }
+
+// Binning key of cell (cx, cy, cz) (inside the grid): super-block-major, then the
+// block within the super-block, then the cell within the block, x fastest.
+__host__ __device__ inline uint32_t super_key(int cx, int cy, int cz, int nsx, int nsy)
+{
+    const uint32_t sb = ((uint32_t)(cz >> 3) * (uint32_t)nsy + (uint32_t)(cy >> 3)) * (uint32_t)nsx + (uint32_t)(cx >> 3);
+    const uint32_t lb = (((cz >> 2) & 1) << 2) | (((cy >> 2) & 1) << 1) | ((cx >> 2) & 1);
+    const uint32_t cell = ((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3);
+    return (sb << SB_KEY_SHIFT) | (lb << 6) | cell;
+}
+
+hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, uint32_t* keys,
+                                  hipStream_t st);
+// Per item of w: the bound of its particles' stored state (what their scatter needs).
+hipError_t launch_item_bounds(int order, const DevParticles& p, const SuperWork& w, const Phys& ph,
+                              hipStream_t st);
+// g2p: one fused substep (G2P from src, P2G into dst); otherwise the P2G of the
+// stored state into dst.  margin: drift margin of the tile in cells (0 or 1).
+hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, const DevParticles& p, const DevGrid& src,
+                        const DevGrid& dst, const SuperWork& w, const Phys& ph, hipStream_t st);
+
+}  // namespace mpm

@@ -30,8 +30,26 @@ def _deviation(x, v, C, F, Jp, x_exact, v_exact):
             "C": float(np.abs(C).max()), "F": float(np.abs(F - eye).max()), "Jp": float(np.abs(Jp - 1.0).max())}
 
 
+_CUBE_ORACLE = {}
+
+
+def _cube_oracle_deviation(P, ps, k, x_exact, v_exact):
+    """The FP32 oracle's deviation from the closed form (all host cores), computed
+    once for every pipeline the test runs."""
+    if k not in _CUBE_ORACLE:
+        print("[fullsize] FP32 oracle (all host cores)", flush=True)
+        o = orc.OracleState.from_particles(ps, np.float32)
+        orc.run_omp(P, o, k)
+        os_ = orc.to_solver_state(P, o)
+        _CUBE_ORACLE[k] = _deviation(os_["x"], os_["v"], os_["C"], os_["F"], os_["Jp"], x_exact, v_exact)
+    return _CUBE_ORACLE[k]
+
+
 @pytest.mark.timeout(600)
-def test_falling_cube_free_fall_closed_form():
+@pytest.mark.parametrize("fused", [1, 2])
+def test_falling_cube_free_fall_closed_form(fused):
+    """fused 1: per-block work items; 2: super-block work items (the cube starts at
+    rest, so its first substep scatters every particle with float atomics)."""
     from mpm_snow_sim2d_amd.solver import GPUSolver
     P = snow_params(3, 256, 5e-5)
     ps = falling_cube(seed=0, n_side=126)
@@ -42,18 +60,14 @@ def test_falling_cube_free_fall_closed_form():
     v_exact = np.array([0.0, g * k * dt, 0.0])[:, None]
     x_exact = ps.x.astype(np.float64) + np.array([0.0, g * dt * dt * k * (k + 1) / 2, 0.0])[:, None]
 
-    print(f"[fullsize] falling cube n={ps.n} substeps={k}: device", flush=True)
-    with GPUSolver(P, ps, Options(sort_every=16, tile_particles=512)) as s:
+    print(f"[fullsize] falling cube n={ps.n} substeps={k} fused={fused}: device", flush=True)
+    opts = Options(sort_every=16, fused=fused, tile_particles=512 if fused == 1 else 0)
+    with GPUSolver(P, ps, opts) as s:
         s.Step(k)
         st = s.get_state()
         assert s.check_finite() == 0
     dev = _deviation(st["x"], st["v"], st["C"], st["F"], st["Jp"], x_exact, v_exact)
-
-    print("[fullsize] FP32 oracle (all host cores)", flush=True)
-    o = orc.OracleState.from_particles(ps, np.float32)
-    orc.run_omp(P, o, k)
-    os_ = orc.to_solver_state(P, o)
-    ref = _deviation(os_["x"], os_["v"], os_["C"], os_["F"], os_["Jp"], x_exact, v_exact)
+    ref = _cube_oracle_deviation(P, ps, k, x_exact, v_exact)
     floor = {"v": 1e-6, "x": 1e-6, "C": 1e-4, "F": 1e-6, "Jp": 1e-6}
     for key in dev:
         print(f"    {key:2s} device {dev[key]:.2e}  FP32 oracle {ref[key]:.2e}", flush=True)
@@ -102,7 +116,8 @@ def test_batch64_scenes_match_single_scene_runs():
 
 
 @pytest.mark.timeout(600)
-def test_large_spheres_rigid_motion_closed_form():
+@pytest.mark.parametrize("fused", [1, 3])
+def test_large_spheres_rigid_motion_closed_form(fused):
     """The large workload at full size (16M particles in four spheres, 512^3 grid,
     dt = 2.5e-5) before any two spheres can interact (surfaces start >= 1 cm apart,
     shared grid nodes need < 3 cells = 5.9 mm, closing speed <= 8 sqrt(3) m/s: 8
@@ -115,8 +130,8 @@ def test_large_spheres_rigid_motion_closed_form():
     P = snow_params(3, 512, 2.5e-5)
     ps = colliding_spheres(seed=LARGE_SEED)
     k = 8
-    print(f"[fullsize] large: {ps.n} particles, 512^3, {k} substeps", flush=True)
-    with GPUSolver(P, ps, Options(sort_every=32, tile_particles=256)) as s:
+    print(f"[fullsize] large: {ps.n} particles, 512^3, {k} substeps, fused={fused}", flush=True)
+    with GPUSolver(P, ps, Options(sort_every=32, fused=fused, tile_particles=256 if fused == 1 else 0)) as s:
         s.Step(k)
         st = s.get_state()
         assert s.check_finite() == 0

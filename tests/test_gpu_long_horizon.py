@@ -97,7 +97,8 @@ def test_snowballs2d_aggregates_through_collision():
 
 
 @pytest.mark.timeout(900)
-def test_large_spheres_momentum_through_contact():
+@pytest.mark.parametrize("fused", [1, 2, 3])
+def test_large_spheres_momentum_through_contact(fused):
     from mpm_snow_sim2d_amd.scenes import LARGE_SEED, colliding_spheres
     from mpm_snow_sim2d_amd.solver import GPUSolver
     P = snow_params(3, 512, 2.5e-5)
@@ -110,8 +111,8 @@ def test_large_spheres_momentum_through_contact():
     dt = float(np.float32(P.DELTA_TIME))
     g = float(np.float32(P.GRAVITY))
     wall = P.BORDER_MARGIN * float(P.dx)
-    _flush(f"[long] large spheres n={ps.n}: 400 substeps through first contact")
-    with GPUSolver(P, ps, Options(sort_every=32, tile_particles=512)) as s:
+    _flush(f"[long] large spheres n={ps.n}: 400 substeps through first contact, fused={fused}")
+    with GPUSolver(P, ps, Options(sort_every=32, fused=fused, tile_particles=512 if fused == 1 else 0)) as s:
         done = 0
         for k in (100, 200, 300, 400):
             s.Step(k - done)
@@ -133,7 +134,8 @@ def test_large_spheres_momentum_through_contact():
 
 
 @pytest.mark.timeout(900)
-def test_falling_cube_free_fall_invariants_1000_substeps():
+@pytest.mark.parametrize("fused", [1, 2])
+def test_falling_cube_free_fall_invariants_1000_substeps(fused):
     from mpm_snow_sim2d_amd.scenes import falling_cube
     from mpm_snow_sim2d_amd.solver import GPUSolver
     P = snow_params(3, 256, 5e-5)
@@ -142,8 +144,8 @@ def test_falling_cube_free_fall_invariants_1000_substeps():
     M = m.sum()
     dt = float(np.float32(P.DELTA_TIME))
     g = float(np.float32(P.GRAVITY))
-    _flush(f"[long] falling cube n={ps.n}: 1000 substeps of free fall")
-    with GPUSolver(P, ps, Options(sort_every=32, tile_particles=512)) as s:
+    _flush(f"[long] falling cube n={ps.n}: 1000 substeps of free fall, fused={fused}")
+    with GPUSolver(P, ps, Options(sort_every=32, fused=fused, tile_particles=512 if fused == 1 else 0)) as s:
         done = 0
         for k in (250, 500, 1000):
             s.Step(k - done)

@@ -1,0 +1,152 @@
+"""The super-block pipeline (mpm_options.fused = 2 or 3: 3D work items of 2x2x2
+grid blocks, one pass per particle, fixed-point scales from the previous substep's
+bounds) against the oracle, through the C-ABI, under the same bounds as the
+per-block pipeline (tests/test_gpu_parity.py): 20 substeps of every 3D scene
+against the FP32 oracle and the FP64 twin, max 1e-3 / median 1e-5 in natural
+units (x: median 1e-4 cells); the binning keys bit-exact against a CPU stable
+sort of the restated key; refusal of 2D and batched solvers."""
+import numpy as np
+import pytest
+
+from mpm_snow_sim2d_amd.params import Options
+from tests.helpers import make_case, state_errors, super_keys
+
+pytestmark = pytest.mark.gpu
+
+CASES3D = ["snow3d_quad_sticky", "snow3d_quad_coulomb", "ref3d_cubic_coulomb", "cpu3d_apic_sticky"]
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("fused", [2, 3])
+@pytest.mark.parametrize("case", CASES3D)
+def test_super_substeps_match_oracle(orc, case, fused):
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case(case)
+    nsub = 20
+    _flush(f"[super] {case} fused={fused} n={ps.n} steps={nsub}")
+    with GPUSolver(P, ps, Options(fused=fused, sort_every=8, tile_particles=0)) as s:
+        s.Step(nsub)
+        gpu = s.get_state()
+        assert s.check_finite() == 0
+    o32 = orc.run(P, orc.OracleState.from_particles(ps, np.float32), nsub)
+    o64 = orc.run(P, orc.OracleState.from_particles(ps, np.float64), nsub)
+    e32 = state_errors(gpu, orc.to_solver_state(P, o32), P.dx)
+    e64 = state_errors(gpu, orc.to_solver_state(P, o64), P.dx)
+    _flush(f"    gpu-vs-f32 {e32}\n    gpu-vs-f64 {e64}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        for e in (e32, e64):
+            assert e[k][0] < 1e-3, (k, e[k])
+            assert e[k][1] < (1e-4 if k == "x" else 1e-5), (k, e[k])
+
+
+def test_super_from_rest_matches_oracle(orc):
+    """A body at rest has a zero bound: its first fused substep's scale comes from
+    the momentum gravity adds in one substep alone; the substeps after it use the
+    bounds that substep wrote."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("snow3d_quad_sticky")
+    ps.v[:] = 0.0
+    nsub = 20
+    _flush(f"[super rest] n={ps.n} steps={nsub}")
+    with GPUSolver(P, ps, Options(fused=2, sort_every=8)) as s:
+        s.Step(nsub)
+        gpu = s.get_state()
+    o64 = orc.to_solver_state(P, orc.run(P, orc.OracleState.from_particles(ps, np.float64), nsub))
+    err = state_errors(gpu, o64, P.dx)
+    _flush(f"    gpu-vs-f64 {err}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert err[k][0] < 1e-3, (k, err[k])
+        assert err[k][1] < (1e-4 if k == "x" else 1e-5), (k, err[k])
+
+
+def test_super_stale_binning_takes_float_path(orc):
+    """Binning only once (sort_every 0): the two bodies move 0.04 cells per substep,
+    so after ~12 substeps particles leave their work item's tile (drift margin of
+    half a cell) and scatter with global float atomics; results stay within the
+    oracle bounds."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("ref3d_cubic_coulomb")
+    nsub = 20
+    _flush(f"[super stale] n={ps.n} steps={nsub}")
+    with GPUSolver(P, ps, Options(fused=2, sort_every=0)) as s:
+        s.Step(nsub)
+        gpu = s.get_state()
+    o64 = orc.to_solver_state(P, orc.run(P, orc.OracleState.from_particles(ps, np.float64), nsub))
+    err = state_errors(gpu, o64, P.dx)
+    _flush(f"    gpu-vs-f64 {err}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert err[k][0] < 1e-3, (k, err[k])
+        assert err[k][1] < (1e-4 if k == "x" else 1e-5), (k, err[k])
+
+
+@pytest.mark.parametrize("case", ["snow3d_quad_sticky", "ref3d_cubic_coulomb"])
+def test_super_matches_per_block_pipeline(case):
+    """Both tiled pipelines from one state, 24 substeps with binning every 4 (six
+    re-binnings, each re-deriving the items' bounds), under the 20-substep oracle
+    bounds: they differ only in how the grid sums are rounded."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case(case)
+    _flush(f"[super vs block] {case} n={ps.n}")
+    out = {}
+    for fused in (1, 2):
+        with GPUSolver(P, ps, Options(fused=fused, sort_every=4)) as s:
+            s.Step(24)
+            out[fused] = s.get_state()
+    err = state_errors(out[2], out[1], P.dx)
+    _flush(f"    {err}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert err[k][0] < 1e-3, (k, err[k])
+        assert err[k][1] < (1e-4 if k == "x" else 1e-5), (k, err[k])
+
+
+@pytest.mark.parametrize("case", ["snow3d_quad_sticky", "ref3d_cubic_coulomb"])
+def test_super_sort_keys_bit_exact(case):
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case(case)
+    _flush(f"[super sort] {case} n={ps.n}")
+    with GPUSolver(P, ps, Options(fused=2)) as s:
+        s.sort()
+        keys, perm = s.get_sort()
+        st = s.get_state()
+    want = super_keys(P, ps.x)
+    order = np.argsort(want, kind="stable")
+    assert np.array_equal(keys, want[order])
+    assert np.array_equal(perm, order.astype(np.uint32))
+    assert np.array_equal(st["x"], ps.x.astype(np.float32))
+
+
+def test_super_after_stage_calls_and_state_changes(orc):
+    """Stage calls and set_state between fused steps: the fused pipeline re-derives
+    its items' bounds from the state as it is (here the velocities are scaled by 50
+    between steps, far beyond the previous bounds' headroom)."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("snow3d_quad_sticky")
+    _flush(f"[super stage] n={ps.n}")
+    with GPUSolver(P, ps, Options(fused=2, sort_every=8)) as s, \
+            GPUSolver(P, ps, Options(fused=0)) as r:
+        for solver in (s, r):
+            solver.Step(5)
+            st = solver.get_state()
+            st["v"] = st["v"] * np.float32(50.0)
+            solver.set_state(v=st["v"])
+            solver.Step(5)
+        a, b = s.get_state(), r.get_state()
+    err = state_errors(a, b, P.dx)
+    _flush(f"    {err}")
+    for k in ("x", "v", "C", "F", "Jp"):
+        assert err[k][0] < 1e-3, (k, err[k])
+
+
+def test_super_refuses_2d_and_batches():
+    from mpm_snow_sim2d_amd.solver import BatchSolver, GPUSolver, MPMError
+    P2, ps2 = make_case("snow2d_quad_sticky")
+    with pytest.raises(MPMError):
+        GPUSolver(P2, ps2, Options(fused=2))
+    P3, ps3 = make_case("snow3d_quad_sticky")
+    with pytest.raises(MPMError):
+        BatchSolver([(P3, ps3), (P3, ps3)], Options(fused=3))
+    with pytest.raises(MPMError):
+        GPUSolver(P3, ps3, Options(fused=4))

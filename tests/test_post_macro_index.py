@@ -84,8 +84,11 @@ int main(int argc, char** argv)
         int e = s + 1;
         while (e < n && (sk[e] >> 6) == b) ++e;
         const int len = e - s;
-        for (int c = 0; c * tile_p < len; ++c) {
-            it_blk.push_back((int)b); it_first.push_back(s + c * tile_p); it_cnt.push_back(std::min(tile_p, len - c * tile_p));
+        const int nch = (len + tile_p - 1) / tile_p;
+        for (int c = 0; c < nch; ++c) {
+            const int lo = (int)((long long)c * len / nch), hi = (int)((long long)(c + 1) * len / nch);
+            CHECK(hi - lo <= tile_p, "work item larger than tile_p");
+            it_blk.push_back((int)b); it_first.push_back(s + lo); it_cnt.push_back(hi - lo);
         }
     }
     CHECK((long)it_blk.size() <= max_items, "more work items than max_items");
