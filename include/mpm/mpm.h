@@ -59,8 +59,13 @@ typedef struct mpm_particles {
 /* Solver options beyond the physics parameters (the reference has none). */
 typedef struct mpm_options {
     int sort_every;     /* re-bin and reorder particles every k substeps of mpm_step; 0 = never */
-    int fused;          /* 1: mpm_step runs the fused G2P+P2G pipeline; 0: the four reference stages */
-    int tile_particles; /* particles per work item of the tiled kernels, 1..512 (0 = 512)         */
+    int fused;          /* 0: the four reference stages; 1: the fused G2P+P2G pipeline over work
+                         * items of one grid block; 2 / 3: the fused pipeline over work items of
+                         * 2x2x2 grid blocks (one 3D scene only; MPM_E_INVALID otherwise) whose
+                         * tile holds particles that drifted up to 0.5 / 1.5 cells since the
+                         * last binning (farther ones scatter with float atomics) */
+    int tile_particles; /* particles per work item at most: 1..512 for fused 1 (0 = 512),
+                         * 1..2048 for fused 2 and 3 (0 = 2048); longer runs split evenly     */
 } mpm_options;
 
 /* Parameters.h defaults (src/shared/Parameters.h:30-84) for dim 2 or 3. */
@@ -94,9 +99,10 @@ int mpm_grid_update(mpm_solver* s);
 int mpm_g2p(mpm_solver* s);
 int mpm_reset_grid(mpm_solver* s);
 
-/* Particle binning: key = cell id in block-major order, stable LSD radix sort,
- * reorder of every particle array.  No reference counterpart (the reference
- * scatters in particle order, shaders/2d/p2g.glsl:88-90). */
+/* Particle binning: key = cell id in block-major order (with fused 2 or 3:
+ * super-block-major, then block, then cell), stable LSD radix sort, reorder of every
+ * particle array.  No reference counterpart (the reference scatters in particle
+ * order, shaders/2d/p2g.glsl:88-90). */
 int mpm_sort(mpm_solver* s);
 /* After mpm_sort: the sorted keys and, for each sorted slot, the original
  * particle index (n entries each; either pointer may be NULL). */
@@ -177,9 +183,11 @@ int mpm_set_graph(mpm_solver* s, int enabled);
 
 /* Autotuner (no reference counterpart): times trial_substeps of the fused pipeline
  * (rounded up to whole binning periods) on fresh solvers built from the same input,
- * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32, 64}, and
- * returns the fastest options and their time per substep.  nscenes = 1 tunes a
- * single scene (params/init point to one element).  The input is not modified. */
+ * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32, 64} with
+ * fused = 1, and for one 3D scene also fused in {2, 3} x tile_particles in
+ * {1024, 2048} x sort_every in {8, 16, 32, 64}, and returns the fastest options and
+ * their time per substep.  nscenes = 1 tunes a single scene (params/init point to
+ * one element).  The input is not modified. */
 int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,
                  int use_graph, mpm_options* best, double* best_ms_per_substep);
 

@@ -760,14 +760,20 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
         }
         for (int pe = tid; pe < TN * NC; pe += 256) {
             const int e = pe / NC, c = pe - e * NC;
-            int acc;
+            int acc, mass;
             if constexpr (PACKED) {
                 const unsigned long long sum = s_acc[c >> 1][e];
                 acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
+                mass = c == 0 ? acc : unpack_lo(s_acc[0][e]);
             } else {
                 acc = s_acc32[c][e];
+                mass = c == 0 ? acc : s_acc32[0][e];
             }
-            if (acc == 0) continue;
+            // a node whose mass rounded to zero flushes no momentum either: mass and
+            // momentum round independently, and a momentum unit over a float-path mass
+            // far below the mass unit (a direct scatter's tiny stencil weight) would
+            // give the node any velocity
+            if (acc == 0 || mass == 0) continue;
             int t[3];
             tile_coords<DIM>(e, t);
             int i = o[0] + t[0], j = o[1] + t[1], k = o[2] + t[2];

@@ -14,15 +14,19 @@
 //     the bound the same particles had one substep earlier (written by this kernel
 //     at the end of each item, and after each binning by k_item_bounds from the
 //     stored state, which is exactly what the previous substep scattered), plus the
-//     momentum gravity adds in one substep, with a factor 2 of headroom.  A particle whose bound exceeds it this substep scatters
-//     with global float atomics instead, so results never depend on the headroom.
+//     momentum gravity adds in one substep.  A particle whose bound exceeds the
+//     scale's range this substep scatters with global float atomics instead, so
+//     results never depend on how well the bound predicted it.
 //
 // Fixed-point accumulation as in k_tiled: per component an integer sum in LDS, two
 // components per 64-bit add.  With cnt particles in the item a node receives at
-// most cnt contributions, each |w * value| * 2^e <= 0.42 * 2^cap (w <= 0.75^3 for
-// the quadratic, (2/3)^3 for the cubic stencil), with cap = min(31 - ceil(log2 cnt), 21),
-// so each 32-bit half of a node's sum stays below 2^31, and every contribution is
-// below 2^22 as the rounding trick of fixed_bits needs.
+// most cnt contributions, each |w * value| * 2^e <= wmax * 2^cap rounded (w <=
+// 0.75^3 for the quadratic, (2/3)^3 for the cubic stencil); cap (fixed_cap) is the
+// largest for which cnt of them stay below 2^31, so each 32-bit half of a node's sum
+// cannot overflow, and every contribution is below 2^22 as the rounding trick of
+// fixed_bits needs.  For 512 to 2048 particles cap is 23 to 21, and the rounding of
+// a contribution is at most 2^-(cap+1) of the scale's power of two, 2^-cap of the
+// item's bound: the per-block kernel's resolution for the same item sizes.
 #include "mpm_fused_kernel.hpp"
 #include "mpm_super.hpp"
 
@@ -57,14 +61,27 @@ __device__ inline float scatter_inputs(const Phys& ph, float pm, float vol, cons
     return bp;
 }
 
-// 2^e with (2 * bound) * 2^e < 2^cap; a zero or non-finite bound gives 2^126, so
-// that every particle with a nonzero contribution takes the float path.
+// 2^e with bound * 2^e in [2^(cap-1), 2^cap): the power-of-two rounding leaves the
+// item between 1x and 2x of headroom over the bound.  A zero or non-finite bound
+// gives 2^126, so that every particle with a nonzero contribution takes the float
+// path.
 __device__ inline float headroom_scale(float bound, int cap)
 {
     if (!(bound > 0.f) || !(bound < 1.0e37f)) return ldexpf(1.f, 126);
     int k;
-    frexpf(2.f * bound, &k);                 // 2 bound < 2^k
+    frexpf(bound, &k);                       // bound < 2^k
     return ldexpf(1.f, min(max(cap - k, -126), 126));
+}
+
+// The largest cap <= 23 for which cnt contributions of at most wmax * 2^cap + 1/2
+// each (a stencil weight times a value within 2^cap, rounded) sum below 2^31 (with a
+// margin of 2^-10 for the float evaluation); 23 keeps every contribution below 2^22
+// as fixed_bits needs.
+__device__ inline int fixed_cap(int cnt, float wmax)
+{
+    int cap = 23;
+    while (cap > 0 && (float)cnt * (wmax * ldexpf(1.f, cap) + 0.5f) >= 2145386496.f) --cap;
+    return cap;
 }
 
 }  // namespace
@@ -185,9 +202,9 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 
         // ---- the item's fixed-point scales, from its particles' bounds one substep
         // earlier plus the momentum gravity adds in a substep (so that a body at
-        // rest has a scale for its first substep), with a factor 2 of headroom
-        const int lg = cnt > 1 ? 32 - __clz(cnt - 1) : 0;     // ceil(log2 cnt)
-        const int cap = min(31 - lg, 21);
+        // rest has a scale for its first substep)
+        // largest stencil weight: 0.75^3 (quadratic), (2/3)^3 (cubic), rounded up
+        const int cap = fixed_cap(cnt, ORDER == 3 ? 0.2963f : 0.4219f);
         const float lim = ldexpf(1.f, cap);
         const float2 prev = work.bound[it];
         const float scale_m = headroom_scale(prev.x, cap);
@@ -542,7 +559,10 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             const int e = pe >> 2, c = pe & 3;
             const unsigned long long sum = s_acc[c >> 1][e];
             const int acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
-            if (acc == 0) continue;
+            // a node whose mass rounded to zero flushes no momentum either: mass and
+            // momentum round independently, and a momentum unit over a float-path
+            // mass far below the mass unit would give the node any velocity
+            if (acc == 0 || (c != 0 && unpack_lo(s_acc[0][e]) == 0)) continue;
             const int i = o[0] + e % T, j = o[1] + (e / T) % T, k = o[2] + e / (T * T);
             if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, i, j, k)]) + c;

@@ -103,7 +103,10 @@ def snow_params(dim: int, cells: int, dt: float, spline_order: int = 2,
 
 @dataclass
 class Options:
-    """Solver options beyond the physics (``mpm_options``)."""
+    """Solver options beyond the physics (``mpm_options``): binning period; pipeline
+    (0 the four reference stages, 1 fused over one-block work items, 2 / 3 fused
+    over 2x2x2-block work items with a drift margin of 0.5 / 1.5 cells, one 3D scene
+    only); particles per work item at most (0: the pipeline's maximum)."""
     sort_every: int = 16
     fused: int = 1
     tile_particles: int = 512
