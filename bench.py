@@ -52,10 +52,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles() -> float | None:
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "g2p2g_falling_cube_traffic.txt")
+def traffic_from_profiles(superblock: bool = False) -> float | None:
+    """HBM bytes per launch of the fused kernel (the per-block kernel, or with
+    superblock the super-block kernel) from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    name = "g2p2g_superblock_falling_cube_traffic.txt" if superblock else "g2p2g_falling_cube_traffic.txt"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -492,7 +494,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-                     "traffic": traffic_from_profiles() if wl == "cube" else None,
+                     "traffic": traffic_from_profiles(opts.fused >= 2) if wl == "cube" else None,
                      "avg_launch_us": round(avg_k * 1e6, 3),
                      "grid_update_avg_us": round(ms_gu / max(ngu, 1) * 1e3, 3),
                      "sort_avg_us": round(ms_sort / max(nsort, 1) * 1e3, 3)},

@@ -36,11 +36,12 @@ def test_cpu_baseline_object():
 
 
 def test_traffic_from_committed_profile():
-    t = bench.traffic_from_profiles()
-    assert t is not None
-    # the summary's traffic is the scaled FETCH_SIZE + WRITE_SIZE of the fused kernel:
-    # at least the 200 B/particle it must move, and not absurdly more
-    assert 2_000_376 * 200 <= t <= 4 * 2_000_376 * 200
+    for superblock in (False, True):
+        t = bench.traffic_from_profiles(superblock)
+        assert t is not None
+        # the summary's traffic is the scaled FETCH_SIZE + WRITE_SIZE of the fused
+        # kernel: at least the 200 B/particle it must move, and not absurdly more
+        assert 2_000_376 * 200 <= t <= 4 * 2_000_376 * 200
 
 
 def test_more_than_one_gpu_is_refused():
