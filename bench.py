@@ -232,8 +232,13 @@ def learned(args) -> dict:
         for mode in (ADVANCE_NETWORK, ADVANCE_BALLISTIC):
             sur.set_macro(args.macro)
             r = rollout(sur, Pq, ps, args.rollout, k, mode, tol=args.tol, options=opts_plain)
-            err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean())
-            res[mode].append({"wall": r["wall"], "err": err, "counts": r["counts"], "bad": r["nonfinite"]})
+            # a rollout that a failed residual test stopped early has no error at the
+            # rollout's end: it counts as failed
+            err = float(np.linalg.norm(r["x"] - ref["x"], axis=1).mean()) if not r["stopped"] else float("nan")
+            if r["stopped"]:
+                log(f"[learned] scene {i} mode {mode}: {r['stopped']}")
+            res[mode].append({"wall": r["wall"], "err": err, "counts": r["counts"], "bad": r["nonfinite"],
+                              "stopped": bool(r["stopped"])})
         # the same network with the other macro-step forms (which field moves the
         # particles, which advances F_E)
         for fl in variants:
@@ -278,6 +283,9 @@ def learned(args) -> dict:
             "train_samples": n_train, "train_steps": train_steps, "batch": args.batch,
             "plain_options": {"sort_every": opts_plain.sort_every, "tile_particles": opts_plain.tile_particles,
                               "hipgraph": True},
+            "learned_rollout": "first macro step through mpm_surrogate_advance, the rest replayed from one "
+                               "recorded hipGraph of the macro step with the residual test on the device",
+            "rollouts_stopped": int(sum(r.get("stopped", False) for r in res[ADVANCE_NETWORK])),
             "generate_s": round(gen_s, 2), "train_s": round(train_s, 2),
             "heldout_loss": round(val_loss, 5), "heldout_loss_ballistic": round(base_loss, 5),
             "validation_loss_per_chunk": [round(v, 5) for v in curve],

@@ -277,6 +277,18 @@ int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters);
 int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol, int* used,
                           float* residual);
 
+/* n_macro macro steps of k_substeps each, as n_macro calls of mpm_surrogate_advance
+ * would take them (no reference counterpart): until the solver has taken a macro
+ * step, each goes through that call (a failed residual test runs plain substeps);
+ * the rest replay one recorded hipGraph of the macro step whose residual test runs
+ * on the device, so the host does not wait between them.  The first of those whose
+ * test fails leaves the particles as they were before it and the call returns
+ * MPM_E_STATE.  *n_network: macro steps taken (network or ballistic); *n_plain:
+ * plain fallbacks; *max_residual: the largest residual of the macro steps taken.
+ * MPM_ADVANCE_PLAIN runs k_substeps * n_macro plain substeps. */
+int mpm_surrogate_rollout(mpm_surrogate* g, mpm_solver* s, int k_substeps, int n_macro, int mode, float tol,
+                          int* n_network, int* n_plain, float* max_residual);
+
 /* Average device time (ms) of `iters` network passes on nb images of the batch
  * buffers: forward only (train = 0) or forward + loss + backward (train = 1). */
 int mpm_surrogate_time(mpm_surrogate* g, int nb, int iters, int train, double* ms);

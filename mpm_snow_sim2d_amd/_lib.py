@@ -36,7 +36,7 @@ EXPORTS = [
     "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
     "mpm_surrogate_set_params", "mpm_surrogate_set_scale", "mpm_surrogate_set_conv", "mpm_surrogate_set_macro", "mpm_surrogate_set_targets", "mpm_surrogate_forward", "mpm_surrogate_gradient",
     "mpm_surrogate_record", "mpm_surrogate_get_sample", "mpm_surrogate_train", "mpm_surrogate_baseline_loss",
-    "mpm_surrogate_advance", "mpm_surrogate_time",
+    "mpm_surrogate_advance", "mpm_surrogate_rollout", "mpm_surrogate_time",
     "mpm_save_snapshot", "mpm_snapshot_info", "mpm_load_snapshot",
     "mpm_camera_default", "mpm_render_frame", "mpm_write_png",
 ]
@@ -129,6 +129,7 @@ def load() -> ctypes.CDLL:
         "mpm_surrogate_train": (i32, [vp, vp, i32, ctypes.c_float, vp]),
         "mpm_surrogate_baseline_loss": (i32, [vp, vp, i32, vp]),
         "mpm_surrogate_advance": (i32, [vp, vp, i32, i32, ctypes.c_float, vp, vp]),
+        "mpm_surrogate_rollout": (i32, [vp, vp, i32, i32, i32, ctypes.c_float, vp, vp, vp]),
         "mpm_surrogate_time": (i32, [vp, i32, i32, i32, P(ctypes.c_double)]),
         "mpm_save_snapshot": (i32, [vp, ctypes.c_char_p]),
         "mpm_snapshot_info": (i32, [ctypes.c_char_p, P(Params), P(i64), P(i64)]),

@@ -472,10 +472,12 @@ __global__ void __launch_bounds__(256) k_pack(const float* __restrict__ data, co
 __global__ void __launch_bounds__(256) k_macro_g2p(DevParticles P, int first, int count, Phys ph, int h, int w,
                                                   const float* __restrict__ sample, const float* __restrict__ outv,
                                                   const float* __restrict__ outu, float d_ref, float T, float ug,
-                                                  int flags)
+                                                  int flags, const int* __restrict__ gate)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= count) return;
+    // a rollout whose residual test has failed leaves the particles as they are
+    if (gate && gate[0] != 0) return;
     const int p = first + k, n = P.n;
     const size_t hw = (size_t)h * w;
     const float xp[2] = {P.x[p], P.x[n + p]};
@@ -572,6 +574,34 @@ __global__ void k_copy_info(SlotInfo* dst, const SlotInfo* src) { *dst = *src; }
 
 }  // namespace
 
+// The residual test of a macro step on the device (the host form is in
+// mpm_surrogate_advance): the live slot's share of the scene's mass in the wall band
+// or outside the image against tol.  gate[2] counts the macro steps taken through
+// the gate; the first one that fails is recorded in gate[0] (1-based) and from then on
+// the gated kernels leave the particles alone; gate[1] holds the largest residual seen
+// before it (as the bits of a non-negative float, so integer max orders them).
+__global__ void k_residual_gate(const SlotInfo* __restrict__ info, float tol, int* __restrict__ gate)
+{
+    if (gate[0] != 0) return;
+    const SlotInfo I = *info;
+    const float M = I.particle_mass > 0.f ? I.particle_mass : 1.f;
+    const float res = fmaxf(I.band_mass / M, fabsf(M - I.grid_mass) / M);
+    const int step = gate[2] + 1;
+    gate[2] = step;
+    if (!(res <= tol)) {
+        gate[0] = step;
+        return;
+    }
+    gate[1] = max(gate[1], __float_as_int(res));
+}
+
+__global__ void k_gate_reset(int* gate)
+{
+    gate[0] = 0;
+    gate[1] = 0;
+    gate[2] = 0;
+}
+
 struct mpm_surrogate {
     hipStream_t st = nullptr;
     UNet net;
@@ -598,6 +628,21 @@ struct mpm_surrogate {
     hipEvent_t ev_capture = nullptr;// orders the solver's next work after a capture
     hipEvent_t ev_info = nullptr;   // the live slot's info has reached info_h
     int live() const { return capacity; }
+    // mpm_surrogate_rollout: device residual gate (failed step, largest residual,
+    // steps taken) and the graph of one gated macro step with what it was recorded for
+    int* gate = nullptr;
+    int* gate_h = nullptr;
+    hipGraphExec_t macro_graph = nullptr;
+    const void* graph_solver = nullptr;
+    const void* graph_particles = nullptr;
+    int graph_k = 0, graph_mode = 0, graph_flags = 0, graph_conv = 0;
+    float graph_tol = 0.f;
+    void drop_graph()
+    {
+        if (macro_graph) (void)hipGraphExecDestroy(macro_graph);
+        macro_graph = nullptr;
+        graph_solver = graph_particles = nullptr;
+    }
 };
 
 namespace {
@@ -693,6 +738,38 @@ int upload_slots(mpm_surrogate* g, const int* slots, int nb)
     return MPM_OK;
 }
 
+// One macro step with the residual test on the device, enqueued on the solver's
+// stream (mpm_surrogate_rollout records it into a graph): capture of the live slot,
+// the gate, the network (MPM_ADVANCE_NETWORK) and the gated macro G2P.  The caller
+// has sized the capture's buffers (ensure_xs) and ordered the solver's stream after
+// the surrogate stream's work.
+int enqueue_gated_macro(mpm_surrogate* g, mpm_solver* s, int k_substeps, int mode, float tol)
+{
+    hipStream_t st = s->stream;
+    int r = capture(g, s, 0, g->live(), 0, k_substeps, true);
+    if (r) return r;
+    k_residual_gate<<<1, 1, 0, st>>>(g->info + g->live(), tol, g->gate);
+    const int hw = g->h * g->w;
+    const float* out = nullptr;
+    if (mode == MPM_ADVANCE_NETWORK) {
+        const int ni = g->images();
+        k_pack<<<std::min((ni * hw + 255) / 256, 8192), 256, 0, st>>>(g->data, g->info, g->live_slot_d, 1, ni, hw,
+                                                                      g->d_ref, g->in, g->wm, nullptr);
+        S_TRY(hipGetLastError());
+        S_TRY(g->net.forward(ni, g->in, g->wm, st));
+        out = g->net.out();
+    }
+    const Phys ph = s->scenes_h[0].ph;
+    const float T = (float)k_substeps * ph.dt;
+    const float ug = ph.gravity * ph.dt * ph.dt * (0.5f * (float)k_substeps * (float)(k_substeps + 1));
+    const float* sample = g->data + (size_t)g->live() * NCH * hw;
+    const float* outu = out && g->images() == 2 ? out + (size_t)2 * hw : nullptr;
+    k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out, outu,
+                                                    g->d_ref, T, ug, g->macro_flags, g->gate);
+    S_TRY(hipGetLastError());
+    return MPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -729,6 +806,8 @@ int mpm_surrogate_create(int height, int width, int max_batch, int capacity, uin
         hipMalloc(&g->loss_d, 256) != hipSuccess ||
         hipMalloc(&g->part_g, (hw / 256 + 1) * sizeof(float)) != hipSuccess ||
         hipMalloc(&g->live_slot_d, sizeof(int)) != hipSuccess ||
+        hipMalloc(&g->gate, 4 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&g->gate_h, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&g->info_h, sizeof(SlotInfo), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_solver, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_capture, hipEventDisableTiming) != hipSuccess ||
@@ -751,6 +830,9 @@ int mpm_surrogate_destroy(mpm_surrogate* g)
 {
     if (!g) return MPM_OK;
     if (g->st) (void)hipStreamSynchronize(g->st);
+    g->drop_graph();
+    (void)hipFree(g->gate);
+    if (g->gate_h) (void)hipHostFree(g->gate_h);
     g->net.release();
     (void)hipFree(g->data);
     (void)hipFree(g->info);
@@ -800,6 +882,7 @@ int mpm_surrogate_set_params(mpm_surrogate* g, const float* host)
 int mpm_surrogate_set_macro(mpm_surrogate* g, int flags)
 {
     if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    g->drop_graph();   // the recorded macro step holds the old setting
     if (flags & ~(MPM_MACRO_DISPLACEMENT_X | MPM_MACRO_DISPLACEMENT_F)) return sfail(MPM_E_INVALID, "unknown macro flags");
     g->macro_flags = flags;
     return MPM_OK;
@@ -817,6 +900,7 @@ int mpm_surrogate_set_targets(mpm_surrogate* g, int refine_iters)
 int mpm_surrogate_set_conv(mpm_surrogate* g, int kernel)
 {
     if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    g->drop_graph();   // the recorded macro step holds the old setting
     if (kernel < MPM_CONV_AUTO || kernel > MPM_CONV_LDS) return sfail(MPM_E_INVALID, "unknown convolution kernel");
     g->net.set_conv_kernel(kernel);
     return MPM_OK;
@@ -825,6 +909,7 @@ int mpm_surrogate_set_conv(mpm_surrogate* g, int kernel)
 int mpm_surrogate_set_scale(mpm_surrogate* g, float d_ref)
 {
     if (!g) return sfail(MPM_E_INVALID, "NULL argument");
+    g->drop_graph();   // the recorded macro step holds the old setting
     if (!(d_ref > 0.f)) return sfail(MPM_E_INVALID, "d_ref must be > 0");
     g->d_ref = d_ref;
     return MPM_OK;
@@ -1014,7 +1099,7 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     const float* sample = g->data + (size_t)g->live() * NCH * hw;
     const float* outu = out && g->images() == 2 ? out + (size_t)2 * hw : nullptr;
     k_macro_g2p<<<(s->n + 255) / 256, 256, 0, st>>>(s->dev_particles(), 0, s->n, ph, g->h, g->w, sample, out, outu,
-                                                    g->d_ref, T, ug, g->macro_flags);
+                                                    g->d_ref, T, ug, g->macro_flags, nullptr);
     S_TRY(hipGetLastError());
     // later work on the surrogate's stream (training, recording) waits for this
     // macro step's use of the shared buffers: the live slot and the network output
@@ -1026,6 +1111,94 @@ int mpm_surrogate_advance(mpm_surrogate* g, mpm_solver* s, int k_substeps, int m
     s->after_macro = true;
     s->substeps_done += k_substeps;
     if (used) *used = mode;
+    return MPM_OK;
+}
+
+int mpm_surrogate_rollout(mpm_surrogate* g, mpm_solver* s, int k_substeps, int n_macro, int mode, float tol,
+                          int* n_network, int* n_plain, float* max_residual)
+{
+    if (!g || !s) return sfail(MPM_E_INVALID, "NULL argument");
+    if (s->nscenes != 1) return sfail(MPM_E_INVALID, "rollout runs on a single-scene solver");
+    if (k_substeps < 1 || n_macro < 0) return sfail(MPM_E_INVALID, "k_substeps must be >= 1 and n_macro >= 0");
+    if (mode < MPM_ADVANCE_PLAIN || mode > MPM_ADVANCE_BALLISTIC) return sfail(MPM_E_INVALID, "unknown mode");
+    int nn = 0, np = 0;
+    float worst = 0.f;
+    auto report = [&] {
+        if (n_network) *n_network = nn;
+        if (n_plain) *n_plain = np;
+        if (max_residual) *max_residual = worst;
+    };
+    report();
+    if (n_macro == 0) return MPM_OK;
+    if (mode == MPM_ADVANCE_PLAIN || s->n == 0) {
+        const int r = mpm_step(s, k_substeps * n_macro);
+        if (r == MPM_OK) np = n_macro;
+        report();
+        return r;
+    }
+    // macro steps through the host test until the solver has taken one (a failed
+    // test runs plain substeps only before that)
+    int done = 0;
+    while (done < n_macro && !s->after_macro) {
+        int used = MPM_ADVANCE_PLAIN;
+        float res = 0.f;
+        const int r = mpm_surrogate_advance(g, s, k_substeps, mode, tol, &used, &res);
+        if (r) { report(); return r; }
+        if (used == MPM_ADVANCE_PLAIN) ++np;
+        else { ++nn; worst = std::max(worst, res); }
+        ++done;
+    }
+    if (done == n_macro) { report(); return MPM_OK; }
+
+    // the rest: replays of one recorded macro step with the test on the device
+    int r = check_scene(g, s, 0);
+    if (r) { report(); return r; }
+    if ((r = ensure_xs(g, s))) { report(); return r; }
+    const void* particles = s->dev_particles().x;
+    if (g->macro_graph && (g->graph_solver != s || g->graph_particles != particles || g->graph_k != k_substeps ||
+                           g->graph_mode != mode || g->graph_tol != tol))
+        g->drop_graph();
+    // the solver's stream waits for pending surrogate work (training, recording)
+    S_TRY(hipEventRecord(g->ev_solver, g->st));
+    S_TRY(hipStreamWaitEvent(s->stream, g->ev_solver, 0));
+    if (!g->macro_graph) {
+        S_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        r = enqueue_gated_macro(g, s, k_substeps, mode, tol);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+        if (r) { if (graph) (void)hipGraphDestroy(graph); report(); return r; }
+        if (e != hipSuccess) { report(); return sfail(MPM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); }
+        const hipError_t ei = hipGraphInstantiate(&g->macro_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) { g->macro_graph = nullptr; report(); return sfail(MPM_E_HIP, "hipGraphInstantiate failed"); }
+        g->graph_solver = s;
+        g->graph_particles = particles;
+        g->graph_k = k_substeps;
+        g->graph_mode = mode;
+        g->graph_tol = tol;
+    }
+    k_gate_reset<<<1, 1, 0, s->stream>>>(g->gate);
+    S_TRY(hipGetLastError());
+    for (int i = done; i < n_macro; ++i) S_TRY(hipGraphLaunch(g->macro_graph, s->stream));
+    S_TRY(hipMemcpyAsync(g->gate_h, g->gate, 3 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    // later work on the surrogate's stream waits for the rollout's use of the shared
+    // buffers (live slot, network)
+    S_TRY(hipEventRecord(g->ev_capture, s->stream));
+    S_TRY(hipStreamWaitEvent(g->st, g->ev_capture, 0));
+    S_TRY(hipStreamSynchronize(s->stream));
+    const int failed = g->gate_h[0];
+    const int taken = failed ? failed - 1 : n_macro - done;
+    nn += taken;
+    worst = std::max(worst, __builtin_bit_cast(float, g->gate_h[1]));
+    s->grid_state = mpm::detail::GRID_DIRTY;
+    s->items_valid = false;
+    s->after_macro = true;
+    s->substeps_done += (int64_t)k_substeps * taken;
+    report();
+    if (failed)
+        return sfail(MPM_E_STATE, "the residual test failed at macro step " + std::to_string(done + failed) +
+                                      " of the rollout, after an earlier macro step: the particles hold the state "
+                                      "before it, and plain substeps after a macro step are not enabled");
     return MPM_OK;
 }
 

@@ -175,6 +175,17 @@ class Surrogate:
                                               ctypes.byref(used), ctypes.byref(res)))
         return int(used.value), float(res.value)
 
+    def rollout_steps(self, solver, k: int, n_macro: int, mode: int, tol: float = 1e-3) -> tuple[int, int, float]:
+        """n_macro macro steps as n_macro advance() calls would take them, the ones
+        after the solver's first macro step replayed from one recorded graph with the
+        residual test on the device.  Returns (macro steps taken, plain fallbacks,
+        largest residual); raises MPMError (MPM_E_STATE) when a test fails after a
+        macro step (the particles then hold the state before that step)."""
+        nn, np_, res = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0.0)
+        check(self._lib.mpm_surrogate_rollout(self._h, solver._h, int(k), int(n_macro), int(mode), ctypes.c_float(tol),
+                                              ctypes.byref(nn), ctypes.byref(np_), ctypes.byref(res)))
+        return int(nn.value), int(np_.value), float(res.value)
+
 
 def generate(sur: Surrogate, scenes, substeps: int, k: int = 8, first_slot: int = 0, options=None,
              graph: bool = True, wall_tol: float | None = None) -> list[list[int]]:
@@ -297,26 +308,41 @@ def train_streaming(sur: Surrogate, scene_fn, chunks: int, substeps: int, k: int
 def rollout(sur: Surrogate | None, params, ps, substeps: int, k: int = 8, mode: int = ADVANCE_NETWORK,
             tol: float = 1e-3, options=None, graph: bool = True) -> dict:
     """Advance one scene by `substeps` (a multiple of k) in macro steps; mode
-    ADVANCE_PLAIN runs the plain solver only.  Returns the final positions
-    (n, 2), the wall time, and how many macro steps each mode took."""
+    ADVANCE_PLAIN runs the plain solver only.  With graph, the plain solver replays
+    hipGraphs of its substeps and the macro steps after the first replay a graph of
+    one macro step (mpm_surrogate_rollout); without, every macro step is one
+    advance() call.  Returns the final positions (n, 2), the wall time, and how
+    many macro steps each mode took."""
     from .params import Options
     from .solver import GPUSolver
     s = GPUSolver(params, ps, options or Options(sort_every=k, tile_particles=256))
     if graph:
         s.set_graph(True)
     counts = {ADVANCE_PLAIN: 0, ADVANCE_NETWORK: 0, ADVANCE_BALLISTIC: 0}
+    stopped = None
     s.sync()
     t0 = time.perf_counter()
-    if mode == ADVANCE_PLAIN or sur is None:
-        s.Step(substeps // k * k)
-        counts[ADVANCE_PLAIN] = substeps // k
-    else:
-        for _ in range(substeps // k):
-            used, _ = sur.advance(s, k, mode, tol)
-            counts[used] += 1
+    try:
+        if mode == ADVANCE_PLAIN or sur is None:
+            s.Step(substeps // k * k)
+            counts[ADVANCE_PLAIN] = substeps // k
+        elif graph:
+            nn, np_, _ = sur.rollout_steps(s, k, substeps // k, mode, tol)
+            counts[mode] += nn
+            counts[ADVANCE_PLAIN] += np_
+        else:
+            for _ in range(substeps // k):
+                used, _ = sur.advance(s, k, mode, tol)
+                counts[used] += 1
+    except _lib.MPMError as e:
+        # a residual test failed after a macro step: the rollout stops there
+        if e.code != _lib.MPM_E_STATE:
+            s.close()
+            raise
+        stopped = str(e)
     s.sync()
     wall = time.perf_counter() - t0
     x = s.GetParticlePositions().copy()
     bad = s.check_finite()
     s.close()
-    return {"x": x, "wall": wall, "counts": counts, "nonfinite": bad}
+    return {"x": x, "wall": wall, "counts": counts, "nonfinite": bad, "stopped": stopped}
