@@ -184,7 +184,7 @@ struct Stencil {
 // ---------------------------------------------------------------------------
 // Small matrix helpers (row-major, fully unrolled).
 template <int DIM>
-__device__ inline float det(const float* A)
+__host__ __device__ inline float det(const float* A)
 {
     if constexpr (DIM == 2) return A[0] * A[3] - A[2] * A[1];
     else
@@ -337,6 +337,103 @@ __device__ inline void svd_nonneg(const float* M, float* U, float* sig, float* V
 #pragma unroll
         for (int r = 0; r < DIM; ++r) U[r * DIM + DIM - 1] = -U[r * DIM + DIM - 1];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Elastic states without an SVD.  G2P clamps the singular values of
+// F^ = (I + dt C) F_E to [theta0, theta1] (g2p.glsl:80-93), and the stress is
+// 2 mu (F - R) F^T + lambda (J - 1) J I (p2g.glsl:44-59).  When every singular value
+// already lies in the clamp range and det F > 0, the clamp changes nothing:
+// F_E' = F^, J_P' = J_P, and with F = V R (V = sqrt(F F^T), the left stretch),
+// 2 mu (F - R) F^T = 2 mu (V - I) V, so neither U, Sigma nor V of the SVD is needed.
+// The test: every eigenvalue of B = F F^T (the sigma^2) lies in [lo2, hi2], i.e.
+// B - lo2 I and hi2 I - B are positive definite (their leading principal minors are
+// positive, Sylvester's criterion), and det F > 0 (a reflection has a negative
+// sigma, which the clamp moves).  With lo2 >= 0.9 and hi2 <= 1.1, E = B - I has |E| <= 0.1, and
+// V - I = sqrt(I + E) - I from the Taylor series of sqrt(1 + x) to x^5 is accurate
+// to 21/1024 |E|^6 < 3e-8.  Symmetric 3x3 matrices as (xx, yy, zz, xy, xz, yz).
+struct Sym3 {
+    float xx, yy, zz, xy, xz, yz;
+};
+
+// a b for symmetric a, b that commute (polynomials in one matrix): symmetric too
+__host__ __device__ inline Sym3 sym_mul(const Sym3& a, const Sym3& b)
+{
+    Sym3 c;
+    c.xx = a.xx * b.xx + a.xy * b.xy + a.xz * b.xz;
+    c.yy = a.xy * b.xy + a.yy * b.yy + a.yz * b.yz;
+    c.zz = a.xz * b.xz + a.yz * b.yz + a.zz * b.zz;
+    c.xy = a.xx * b.xy + a.xy * b.yy + a.xz * b.yz;
+    c.xz = a.xx * b.xz + a.xy * b.yz + a.xz * b.zz;
+    c.yz = a.xy * b.xz + a.yy * b.yz + a.yz * b.zz;
+    return c;
+}
+
+__host__ __device__ inline Sym3 sym_add_diag(Sym3 a, float d)
+{
+    a.xx += d;
+    a.yy += d;
+    a.zz += d;
+    return a;
+}
+
+// s (B - c I) positive definite (s = +1) or s (B - c I) with s = -1, i.e. c I - B:
+// the leading principal minors of that symmetric matrix are all positive
+__host__ __device__ inline bool positive_definite(const Sym3& B, float c, float s)
+{
+    const float a11 = s * (B.xx - c), a22 = s * (B.yy - c), a33 = s * (B.zz - c);
+    const float a12 = s * B.xy, a13 = s * B.xz, a23 = s * B.yz;
+    const float m2 = a11 * a22 - a12 * a12;
+    const float m3 = a11 * (a22 * a33 - a23 * a23) - a12 * (a12 * a33 - a23 * a13) + a13 * (a12 * a23 - a22 * a13);
+    return a11 > 0.f && m2 > 0.f && m3 > 0.f;
+}
+
+// The clamp bounds of the test: [theta0^2, theta1^2] within [0.9, 1.1], pulled in
+// by 2^-20 so that the FP32 rounding of B cannot admit a singular value the clamp
+// would move.
+__host__ __device__ inline void elastic_bounds(float theta0, float theta1, float* lo2, float* hi2)
+{
+    *lo2 = fmaxf(theta0 * theta0, 0.9f) * (1.f + 9.5367431640625e-7f);
+    *hi2 = fminf(theta1 * theta1, 1.1f) * (1.f - 9.5367431640625e-7f);
+}
+
+// True when F (row-major 3x3) passes the test; then *vmi = V - I and *J = det F.
+__host__ __device__ inline bool elastic_stretch(const float* F, float lo2, float hi2, Sym3* vmi, float* J)
+{
+    Sym3 B;
+    B.xx = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    B.yy = F[3] * F[3] + F[4] * F[4] + F[5] * F[5];
+    B.zz = F[6] * F[6] + F[7] * F[7] + F[8] * F[8];
+    B.xy = F[0] * F[3] + F[1] * F[4] + F[2] * F[5];
+    B.xz = F[0] * F[6] + F[1] * F[7] + F[2] * F[8];
+    B.yz = F[3] * F[6] + F[4] * F[7] + F[5] * F[8];
+    *J = det<3>(F);
+    if (!(*J > 0.f) || !positive_definite(B, lo2, 1.f) || !positive_definite(B, hi2, -1.f)) return false;
+    const Sym3 E = sym_add_diag(B, -1.f);
+    // sqrt(1 + x) - 1 = x (1/2 + x (-1/8 + x (1/16 + x (-5/128 + x 7/256))))
+    Sym3 P = E;
+    P.xx *= 7.f / 256.f; P.yy *= 7.f / 256.f; P.zz *= 7.f / 256.f;
+    P.xy *= 7.f / 256.f; P.xz *= 7.f / 256.f; P.yz *= 7.f / 256.f;
+    P = sym_add_diag(P, -5.f / 128.f);
+    P = sym_add_diag(sym_mul(E, P), 1.f / 16.f);
+    P = sym_add_diag(sym_mul(E, P), -1.f / 8.f);
+    P = sym_add_diag(sym_mul(E, P), 0.5f);
+    *vmi = sym_mul(E, P);
+    return true;
+}
+
+// Kirchhoff stress of an elastic state from its left stretch: 2 mu h (V - I) V +
+// lambda h (J - 1) J I with (V - I) V = (V - I) + (V - I)^2, h = exp(xi (1 - J_P)).
+__host__ __device__ inline void elastic_kirchhoff(const Sym3& vmi, float J, float tmu, float la, float* tau)
+{
+    const Sym3 q = sym_mul(vmi, vmi);
+    const float sI = la * (J - 1.f) * J;
+    tau[0] = tmu * (vmi.xx + q.xx) + sI;
+    tau[4] = tmu * (vmi.yy + q.yy) + sI;
+    tau[8] = tmu * (vmi.zz + q.zz) + sI;
+    tau[1] = tau[3] = tmu * (vmi.xy + q.xy);
+    tau[2] = tau[6] = tmu * (vmi.xz + q.xz);
+    tau[5] = tau[7] = tmu * (vmi.yz + q.yz);
 }
 
 // Kirchhoff stress of F with polar rotation R = U V^T (shaders/2d/p2g.glsl:44-59):

@@ -362,27 +362,55 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                     Fo[e] = P.F[e * n + p];
                 }
                 matmul<3>(M, Fo, Fh);
-                float U[9], sig[3], V[9], scv[3];
-                svd3(Fh, U, sig, V);
-                float ratio = 1.f, J = 1.f;
+                // an elastic F^ (every singular value inside the clamp range, det > 0)
+                // skips the SVD: F_E' = F^, J_P' = J_P, stress from the left stretch
+                float elo2, ehi2, Jh;
+                elastic_bounds(ph.theta0, ph.theta1, &elo2, &ehi2);
+                Sym3 vmi;
+                if (elastic_stretch(Fh, elo2, ehi2, &vmi, &Jh)) {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    float sv = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
-                    sv = fmaxf(sv, 1e-6f);
-                    scv[d] = sv;
-                    ratio *= sig[d] / sv;
-                    J *= sv;
-                }
+                    for (int e = 0; e < 9; ++e) F[e] = Fh[e];
+                    const float harden = expf(ph.xi * (1.f - Jp));
+                    elastic_kirchhoff(vmi, Jh, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
+                } else {
+                    float U[9], sig[3], V[9], scv[3];
+                    svd3(Fh, U, sig, V);
+                    float ratio = 1.f, J = 1.f;
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        float sacc = (U[r * 3] * scv[0]) * V[c * 3];
-#pragma unroll
-                        for (int qq = 1; qq < 3; ++qq) sacc += (U[r * 3 + qq] * scv[qq]) * V[c * 3 + qq];
-                        F[r * 3 + c] = sacc;
+                    for (int d = 0; d < 3; ++d) {
+                        float sv = fminf(fmaxf(sig[d], ph.theta0), ph.theta1);
+                        sv = fmaxf(sv, 1e-6f);
+                        scv[d] = sv;
+                        ratio *= sig[d] / sv;
+                        J *= sv;
                     }
-                Jp = Jp * ratio;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            float sacc = (U[r * 3] * scv[0]) * V[c * 3];
+#pragma unroll
+                            for (int qq = 1; qq < 3; ++qq) sacc += (U[r * 3 + qq] * scv[qq]) * V[c * 3 + qq];
+                            F[r * 3 + c] = sacc;
+                        }
+                    Jp = Jp * ratio;
+                    // stress of the new state: U diag(2 mu (s-1) s + la (J-1) J) U^T
+                    const float harden = expf(ph.xi * (1.f - Jp));
+                    const float tmu = 2.f * (ph.mu * harden);
+                    const float sI = (ph.lambda * harden) * (J - 1.f) * J;
+                    float tdiag[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) tdiag[d] = tmu * (scv[d] - 1.f) * scv[d] + sI;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            float sacc = U[r * 3] * tdiag[0] * U[c * 3];
+#pragma unroll
+                            for (int qq = 1; qq < 3; ++qq) sacc += U[r * 3 + qq] * tdiag[qq] * U[c * 3 + qq];
+                            tau[r * 3 + c] = sacc;
+                        }
+                }
 #pragma unroll
                 for (int d = 0; d < 3; ++d) { x[d] = nx[d]; v[d] = nv[d]; }
 #pragma unroll
@@ -390,22 +418,6 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 #pragma unroll
                 for (int e = 0; e < 9; ++e) { P.C[e * n + p] = C[e]; P.F[e * n + p] = F[e]; }
                 P.Jp[p] = Jp;
-                // stress of the new state: U diag(2 mu (s-1) s + la (J-1) J) U^T
-                const float harden = expf(ph.xi * (1.f - Jp));
-                const float tmu = 2.f * (ph.mu * harden);
-                const float sI = (ph.lambda * harden) * (J - 1.f) * J;
-                float tdiag[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) tdiag[d] = tmu * (scv[d] - 1.f) * scv[d] + sI;
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        float sacc = U[r * 3] * tdiag[0] * U[c * 3];
-#pragma unroll
-                        for (int qq = 1; qq < 3; ++qq) sacc += U[r * 3 + qq] * tdiag[qq] * U[c * 3 + qq];
-                        tau[r * 3 + c] = sacc;
-                    }
             } else {
                 // stress from the stored state (p2g.glsl:44-59)
                 float F[9], U[9], sig[3], V[9];

@@ -1,0 +1,111 @@
+"""The elastic path of the fused kernels without an SVD (elastic_stretch /
+elastic_kirchhoff in csrc/mpm_device.hpp), compiled for the host from the same
+header the kernels use (hipcc, no device needed), against FP64 NumPy.
+
+When every singular value of F lies in the clamp range [1 - theta_c, 1 + theta_s]
+(shaders/3d/g2p.glsl:80-93) and det F > 0, the clamp leaves F unchanged and the
+Kirchhoff stress 2 mu (F - R) F^T + lambda (J - 1) J I (shaders/3d/p2g.glsl:44-59,
+R the polar rotation) equals 2 mu (V - I) V + lambda (J - 1) J I with V the left
+stretch.  Checked: the test accepts elastic matrices of any rotation and rejects
+every matrix whose clamp would move a singular value (above, below, a reflection);
+V - I from the series against the symmetric square root of F F^T (eigh); the stress
+against the polar-decomposition formula (SVD)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROGRAM = r"""
+#include <cstdio>
+#include "mpm_device.hpp"
+
+int main()
+{
+    float th0, th1, tmu, la;
+    int n;
+    if (std::scanf("%f %f %f %f %d", &th0, &th1, &tmu, &la, &n) != 5) return 2;
+    float lo2, hi2;
+    mpm::elastic_bounds(th0, th1, &lo2, &hi2);
+    for (int i = 0; i < n; ++i) {
+        float F[9];
+        for (int e = 0; e < 9; ++e)
+            if (std::scanf("%f", &F[e]) != 1) return 3;
+        mpm::Sym3 v{};
+        float J = 0.f, tau[9] = {0};
+        const bool ok = mpm::elastic_stretch(F, lo2, hi2, &v, &J);
+        if (ok) mpm::elastic_kirchhoff(v, J, tmu, la, tau);
+        std::printf("%d %.9g %.9g %.9g %.9g %.9g %.9g %.9g", ok ? 1 : 0, v.xx, v.yy, v.zz, v.xy, v.xz, v.yz, J);
+        for (int e = 0; e < 9; ++e) std::printf(" %.9g", tau[e]);
+        std::printf("\n");
+    }
+    return 0;
+}
+"""
+
+
+def _rot(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"),
+                    reason="hipcc not available")
+def test_elastic_stretch_against_polar_decomposition(tmp_path):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    src = tmp_path / "el.hip"
+    src.write_text(PROGRAM)
+    exe = tmp_path / "el"
+    cmd = [hipcc, "-std=c++17", "-O2", "--offload-arch=gfx950", "-I",
+           os.path.join(ROOT, "mpm_snow_sim2d_amd", "csrc"), str(src), "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    th_c, th_s = 2.5e-2, 7.5e-3
+    th0, th1 = 1.0 - th_c, 1.0 + th_s
+    tmu, la = 2 * 5.83e4 * 1.3, 3.89e4 * 1.3
+    rng = np.random.default_rng(3)
+    cases, want_ok = [], []
+    # elastic: singular values strictly inside, arbitrary rotations
+    for _ in range(300):
+        s = rng.uniform(th0 + 2e-4, th1 - 2e-4, 3)
+        cases.append(_rot(rng) @ np.diag(s) @ _rot(rng))
+        want_ok.append(True)
+    cases += [np.eye(3), _rot(rng), np.diag([th0 + 1e-4, 1.0, th1 - 1e-4])]
+    want_ok += [True, True, True]
+    # clamped: a singular value above, below, or a reflection (negative sigma)
+    for s in ([th1 + 1e-3, 1.0, 1.0], [1.0, th0 - 1e-3, 1.0], [1.2, 0.9, 1.0], [1.0, 1.0, -1.0], [0.5, 1, 1],
+              [th1 + 3e-6, 1.0, 1.0], [1.0, 1.0, th0 - 3e-6], [th1 + 3e-6, th0 - 3e-6, 1.0]):
+        cases.append(_rot(rng) @ np.diag(s) @ _rot(rng))
+        want_ok.append(False)
+    cases.append(np.diag([1.0, 1.0, -1.0]))
+    want_ok.append(False)
+    F32 = [np.asarray(c, np.float32) for c in cases]
+    inp = f"{th0!r} {th1!r} {tmu!r} {la!r} {len(F32)}\n" + "\n".join(" ".join(repr(float(v)) for v in c.reshape(-1))
+                                                            for c in F32) + "\n"
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    rows = [list(map(float, ln.split())) for ln in out.stdout.strip().splitlines()]
+    assert len(rows) == len(F32)
+    for F, row, ok_want in zip(F32, rows, want_ok):
+        ok = bool(row[0])
+        # a clamped matrix must never pass; the elastic ones (2e-4 inside) all pass
+        assert ok == ok_want, (F, ok, ok_want)
+        if not ok:
+            continue
+        Fd = F.astype(np.float64)
+        w, Q = np.linalg.eigh(Fd @ Fd.T)
+        V = Q @ np.diag(np.sqrt(w)) @ Q.T
+        vmi = np.array([[row[1], row[4], row[5]], [row[4], row[2], row[6]], [row[5], row[6], row[3]]])
+        assert np.abs(vmi - (V - np.eye(3))).max() < 2e-7
+        J = np.linalg.det(Fd)
+        assert abs(row[7] - J) < 1e-6
+        U, S, Vt = np.linalg.svd(Fd)
+        R = U @ Vt
+        tau = tmu * (Fd - R) @ Fd.T + la * (J - 1.0) * J * np.eye(3)
+        got = np.array(row[8:17]).reshape(3, 3)
+        scale = tmu * 0.05 + abs(la)
+        assert np.abs(got - tau).max() <= 2e-6 * scale, (np.abs(got - tau).max(), scale)

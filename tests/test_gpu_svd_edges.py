@@ -91,9 +91,10 @@ def _oracle(P, ps, F, dtype):
     return s["F"].astype(np.float64), s["Jp"].astype(np.float64), st.x.astype(np.float64)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("dim,fused", [(2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
 def test_svd_edge_cases_match_oracle(dim, fused):
+    """fused 2 (3D super-block items) takes elastic cases through the left stretch
+    without an SVD and the others through svd3."""
     from mpm_snow_sim2d_amd.solver import GPUSolver
     cases = _cases(dim)
     P, ps, F = _scene(dim, cases)
