@@ -100,9 +100,11 @@ int mpm_g2p(mpm_solver* s);
 int mpm_reset_grid(mpm_solver* s);
 
 /* Particle binning: key = cell id in block-major order (with fused 2 or 3:
- * super-block-major, then block, then cell), stable LSD radix sort, reorder of every
- * particle array.  No reference counterpart (the reference scatters in particle
- * order, shaders/2d/p2g.glsl:88-90). */
+ * super-block-major, then block, then cell; the periodic re-binning of mpm_step
+ * then sorts by the super-block number alone, keeping the previous order inside
+ * each super-block), stable LSD radix sort, reorder of every particle array.
+ * mpm_get_sort returns the keys of the last binning.  No reference counterpart
+ * (the reference scatters in particle order, shaders/2d/p2g.glsl:88-90). */
 int mpm_sort(mpm_solver* s);
 /* After mpm_sort: the sorted keys and, for each sorted slot, the original
  * particle index (n entries each; either pointer may be NULL). */

@@ -170,37 +170,35 @@ SuperWork super_work(const mpm_solver* s);
 int do_sort(mpm_solver* s)
 {
     const int n = s->n;
+    // Super-block items: a binning of a state the items no longer describe (the first,
+    // or after mpm_set_state or stage calls) sorts by the full cell key; the periodic
+    // re-binning by the super-block number alone, a stable sort that keeps the order
+    // the last full sort gave inside each super-block (fewer radix passes).
+    const bool cells = !s->super_items || !s->items_valid;
     auto body = [&]() -> hipError_t {
         DevParticles p = s->dev_particles();
         hipError_t e = s->scenes_d      ? launch_cell_keys_batch(s->dim, p.x, n, s->scenes_d, s->p_off_d, s->nscenes,
                                                                  s->keys, s->stream)
-                       : s->super_items ? launch_cell_keys_super(p.x, n, s->ph, s->nsx, s->nsy, s->keys, s->stream)
+                       : s->super_items ? launch_cell_keys_super(p.x, n, s->ph, s->nsx, s->nsy, cells, s->keys,
+                                                                 s->stream)
                                         : launch_cell_keys(s->dim, p.x, n, s->ph, s->dev_grid(0), s->keys, s->stream);
         if (e != hipSuccess) return e;
         bool in_alt = false;
-        e = radix_sort_pairs(s->keys, s->vals, s->keys_alt, s->vals_alt, n, s->key_bits, true,
-                             s->radix_scratch, s->stream, &in_alt);
+        e = radix_sort_pairs(s->keys, s->vals, s->keys_alt, s->vals_alt, n, cells ? s->key_bits : s->key_bits_super,
+                             true, s->radix_scratch, s->stream, &in_alt);
         if (e != hipSuccess) return e;
         s->sorted_keys_in_alt = in_alt;
         const uint32_t* perm = in_alt ? s->vals_alt : s->vals;
         const uint32_t* skeys = in_alt ? s->keys_alt : s->keys;
-        ParticleBuffers& src = s->pb[s->cur_pb];
-        ParticleBuffers& dst = s->pb[1 - s->cur_pb];
-        const int d = s->dim;
-        struct { const float* a; float* b; int c; } fields[] = {
-            {src.x, dst.x, d}, {src.v, dst.v, d}, {src.C, dst.C, d * d}, {src.F, dst.F, d * d},
-            {src.Jp, dst.Jp, 1}, {src.mass, dst.mass, 1}, {src.vol, dst.vol, 1}};
-        for (auto& f : fields) {
-            e = launch_gather_fields(f.a, f.b, perm, n, f.c, s->stream);
-            if (e != hipSuccess) return e;
-        }
-        e = launch_gather_u32(src.pid, dst.pid, perm, n, s->stream);
+        // x, v, C, F, J_P, mass, volume and original index sit one after another
+        // from x in each particle buffer (alloc_particles)
+        e = launch_gather_particles(s->dim, s->pb[s->cur_pb].x, s->pb[1 - s->cur_pb].x, perm, n, s->stream);
         if (e != hipSuccess) return e;
         s->cur_pb = 1 - s->cur_pb;
         // the pair of key/value buffers the sort did not end in is free now
         uint32_t* tmp_a = in_alt ? s->keys : s->keys_alt;
         uint32_t* tmp_b = in_alt ? s->vals : s->vals_alt;
-        e = launch_emit_items(skeys, n, s->opts.tile_particles, s->super_items ? SB_KEY_SHIFT : 6, s->items,
+        e = launch_emit_items(skeys, n, s->opts.tile_particles, s->super_items ? (cells ? SB_KEY_SHIFT : 0) : 6, s->items,
                               s->n_items, s->max_items, tmp_a, tmp_b, s->radix_scratch, s->block_scene_d, s->stream);
         if (e != hipSuccess || !s->super_items) return e;
         // the new items' fixed-point bounds from the reordered state
@@ -561,6 +559,8 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
         nsuper = (int64_t)s->nsx * s->nsy * s->nsz;
         maxkey = (uint64_t)nsuper << SB_KEY_SHIFT;
         if (maxkey > ((uint64_t)1 << 32)) return cleanup_fail(fail(MPM_E_INVALID, "grid too large for 32-bit super-block keys"));
+        s->key_bits_super = 1;
+        while (((uint64_t)1 << s->key_bits_super) < (uint64_t)nsuper) ++s->key_bits_super;
     }
     s->key_bits = 1;
     while (s->key_bits < 32 && ((uint64_t)1 << s->key_bits) < maxkey) ++s->key_bits;

@@ -71,6 +71,7 @@ struct mpm_solver {
     bool super_items = false;
     int super_margin = 0;
     int nsx = 0, nsy = 0, nsz = 0;
+    int key_bits_super = 0;          // bits of a super-block number (periodic re-binning)
     float2* item_bound = nullptr;
     bool items_valid = false;
     bool sorted_keys_in_alt = false;

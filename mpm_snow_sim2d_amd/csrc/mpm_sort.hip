@@ -262,6 +262,26 @@ __global__ void __launch_bounds__(256) k_gather_fields(const float* __restrict__
     for (int c = 0; c < ncomp; ++c) dst[(size_t)c * n + s] = src[(size_t)c * n + q];
 }
 
+// Every particle array in one pass: the solver's particle buffer holds `rows` float
+// rows of n (x, v, C, F, J_P, mass, volume) followed by the u32 row of original
+// indices; dst[r][s] = src[r][perm[s]] for every row (one index load per particle,
+// all of its field loads in flight together).
+template <int ROWS>
+__global__ void __launch_bounds__(256) k_gather_rows(const float* __restrict__ src, float* __restrict__ dst,
+                                                    const uint32_t* __restrict__ perm, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const size_t q = perm[s];
+    float v[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) v[r] = src[(size_t)r * n + q];
+    const uint32_t id = reinterpret_cast<const uint32_t*>(src + (size_t)ROWS * n)[q];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) dst[(size_t)r * n + s] = v[r];
+    reinterpret_cast<uint32_t*>(dst + (size_t)ROWS * n)[s] = id;
+}
+
 __global__ void __launch_bounds__(256) k_gather_u32(const uint32_t* __restrict__ src,
                                                    uint32_t* __restrict__ dst, const uint32_t* perm, int n)
 {
@@ -343,6 +363,16 @@ hipError_t launch_gather_fields(const float* src, float* dst, const uint32_t* pe
 {
     if (n == 0) return hipSuccess;
     k_gather_fields<<<(n + 255) / 256, 256, 0, st>>>(src, dst, perm, n, ncomp);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_particles(int dim, const float* src, float* dst, const uint32_t* perm, int n,
+                                   hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255) / 256), blk(256);
+    if (dim == 3) k_gather_rows<27><<<grid, blk, 0, st>>>(src, dst, perm, n);
+    else          k_gather_rows<15><<<grid, blk, 0, st>>>(src, dst, perm, n);
     return hipGetLastError();
 }
 

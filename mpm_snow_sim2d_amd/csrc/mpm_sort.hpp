@@ -40,6 +40,10 @@ hipError_t launch_gather_fields(const float* src, float* dst, const uint32_t* pe
                                 hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t* src, uint32_t* dst, const uint32_t* perm, int n,
                              hipStream_t st);
+// Reorder a whole particle buffer (2 dim + 2 dim^2 + 3 float rows of n, then the u32
+// row of original indices, contiguous from src / dst) by perm in one launch.
+hipError_t launch_gather_particles(int dim, const float* src, float* dst, const uint32_t* perm, int n,
+                                   hipStream_t st);
 // Work items (block id = key >> shift, first particle, count, scene) from sorted
 // keys, in key order; tmp_runlen and tmp_counts hold n words each, scan_sums
 // n/1024 + 1; block_scene (batch-wide block -> scene) may be NULL for one scene.

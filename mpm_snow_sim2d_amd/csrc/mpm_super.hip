@@ -599,9 +599,10 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 
 // Super-block binning keys: the cell containing the particle, clamped into the
 // grid, numbered super-block-major: key = ((sz * nsy + sy) * nsx + sx) * 512 +
-// (block within the super-block) * 64 + (cell within the block, x fastest).
+// (block within the super-block) * 64 + (cell within the block, x fastest), or with
+// cells = false the super-block number alone.
 __global__ void __launch_bounds__(256) k_cell_keys_super(const float* x, int n, float inv_dx, int W, int H, int D,
-                                                         int nsx, int nsy, uint32_t* keys)
+                                                         int nsx, int nsy, bool cells, uint32_t* keys)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -609,15 +610,15 @@ __global__ void __launch_bounds__(256) k_cell_keys_super(const float* x, int n, 
     int c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[d] = min(max(cell_floor(x[d * n + p] * inv_dx), 0), lim[d]);
-    keys[p] = super_key(c[0], c[1], c[2], nsx, nsy);
+    keys[p] = super_key(c[0], c[1], c[2], nsx, nsy, cells);
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, uint32_t* keys,
-                                  hipStream_t st)
+hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, bool cells,
+                                  uint32_t* keys, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    k_cell_keys_super<<<(n + 255) / 256, 256, 0, st>>>(x, n, ph.inv_dx, ph.W, ph.H, ph.D, nsx, nsy, keys);
+    k_cell_keys_super<<<(n + 255) / 256, 256, 0, st>>>(x, n, ph.inv_dx, ph.W, ph.H, ph.D, nsx, nsy, cells, keys);
     return hipGetLastError();
 }
 

@@ -26,17 +26,20 @@ struct SuperWork {
 __host__ __device__ constexpr int super_count(int nb) { return (nb + SB_BLOCKS - 1) / SB_BLOCKS; }
 
 // Binning key of cell (cx, cy, cz) (inside the grid): super-block-major, then the
-// block within the super-block, then the cell within the block, x fastest.
-__host__ __device__ inline uint32_t super_key(int cx, int cy, int cz, int nsx, int nsy)
+// block within the super-block, then the cell within the block, x fastest; with
+// cells = false the super-block alone (key >> SB_KEY_SHIFT of the full key), so that a
+// stable sort keeps the previous order inside each super-block.
+__host__ __device__ inline uint32_t super_key(int cx, int cy, int cz, int nsx, int nsy, bool cells = true)
 {
     const uint32_t sb = ((uint32_t)(cz >> 3) * (uint32_t)nsy + (uint32_t)(cy >> 3)) * (uint32_t)nsx + (uint32_t)(cx >> 3);
+    if (!cells) return sb;
     const uint32_t lb = (((cz >> 2) & 1) << 2) | (((cy >> 2) & 1) << 1) | ((cx >> 2) & 1);
     const uint32_t cell = ((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3);
     return (sb << SB_KEY_SHIFT) | (lb << 6) | cell;
 }
 
-hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, uint32_t* keys,
-                                  hipStream_t st);
+hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, bool cells,
+                                  uint32_t* keys, hipStream_t st);
 // Per item of w: the bound of its particles' stored state (what their scatter needs).
 hipError_t launch_item_bounds(int order, const DevParticles& p, const SuperWork& w, const Phys& ph,
                               hipStream_t st);

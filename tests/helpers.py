@@ -107,6 +107,12 @@ def super_keys(params: Params, x: np.ndarray) -> np.ndarray:
     return ((sb << 9) | (lb << 6) | cell).astype(np.uint32)
 
 
+def super_block_ids(params: Params, x: np.ndarray) -> np.ndarray:
+    """The super-block number alone (the key of the periodic re-binning of the
+    super-block pipeline): super_keys >> 9."""
+    return (super_keys(params, x) >> 9).astype(np.uint32)
+
+
 def state_errors(a: dict, b: dict, dx: float) -> dict:
     """Max / median absolute differences per field, each in its natural unit: x in
     grid cells, v relative to the largest speed, C relative to the velocity-gradient

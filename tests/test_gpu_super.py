@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from mpm_snow_sim2d_amd.params import Options
-from tests.helpers import make_case, state_errors, super_keys
+from tests.helpers import make_case, state_errors, super_block_ids, super_keys
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +116,26 @@ def test_super_sort_keys_bit_exact(case):
     assert np.array_equal(keys, want[order])
     assert np.array_equal(perm, order.astype(np.uint32))
     assert np.array_equal(st["x"], ps.x.astype(np.float32))
+
+
+def test_super_periodic_rebinning_is_stable_by_super_block():
+    """After the first (full-key) binning, the re-binning of a stepping solver sorts
+    by super-block number alone: keys bit-exact against a CPU stable sort of the
+    restated numbers in the solver's current slot order, which it keeps within each
+    super-block."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("snow3d_quad_sticky")
+    _flush(f"[super resort] n={ps.n}")
+    with GPUSolver(P, ps, Options(fused=2, sort_every=4)) as s:
+        s.Step(6)                          # a full binning, then one by super-block
+        _, perm0 = s.get_sort()            # original index of each slot
+        x = s.get_state()["x"]             # caller order
+        s.sort()
+        keys1, perm1 = s.get_sort()
+    sb = super_block_ids(P, x[:, perm0])
+    order = np.argsort(sb, kind="stable")
+    assert np.array_equal(keys1, sb[order])
+    assert np.array_equal(perm1, perm0[order])
 
 
 def test_super_after_stage_calls_and_state_changes(orc):
