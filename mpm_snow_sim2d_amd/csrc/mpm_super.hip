@@ -103,13 +103,24 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
             const float vol = P.vol[p];
             if (vol == 0.f) continue;                                // p2g.glsl:42
             const float pm = P.mass[p], Jp = P.Jp[p];
-            float v[3], C[9], F[9], U[9], sig[3], V[9], tau[9], mvp[3], A[9];
+            float v[3], C[9], F[9], tau[9], mvp[3], A[9];
 #pragma unroll
             for (int d = 0; d < 3; ++d) v[d] = P.v[d * n + p];
 #pragma unroll
             for (int e = 0; e < 9; ++e) { C[e] = P.C[e * n + p]; F[e] = P.F[e * n + p]; }
-            svd3(F, U, sig, V);
-            kirchhoff<3>(ph, F, U, V, det<3>(F), Jp, tau);
+            // the stress of an elastic state from its left stretch, as the fused
+            // kernel takes it, otherwise through the SVD
+            float elo2, ehi2, J;
+            elastic_bounds(ph.theta0, ph.theta1, &elo2, &ehi2);
+            Sym3 vmi;
+            if (elastic_stretch(F, elo2, ehi2, &vmi, &J)) {
+                const float harden = expf(ph.xi * (1.f - Jp));
+                elastic_kirchhoff(vmi, J, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
+            } else {
+                float U[9], sig[3], V[9];
+                svd3(F, U, sig, V);
+                kirchhoff<3>(ph, F, U, V, det<3>(F), Jp, tau);
+            }
             const float bp = scatter_inputs<ORDER>(ph, pm, vol, v, C, tau, mvp, A);
             if (bp < 1.0e37f && fabsf(pm) < 1.0e37f) {
                 bm = fmaxf(bm, fabsf(pm));
