@@ -218,6 +218,8 @@ SuperWork super_work(const mpm_solver* s)
     w.n_items = s->n_items;
     w.max_items = s->max_items;
     w.bound = s->item_bound;
+    w.ref = s->item_ref;
+    w.shift = s->item_shift;
     w.nsx = s->nsx;
     w.nsy = s->nsy;
     return w;
@@ -570,7 +572,11 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
         return cleanup_fail(fail(MPM_E_HIP, "binning allocation failed"));
     s->max_items = (int)std::min<int64_t>(s->n, s->super_items ? nsuper : s->nblocks) + s->n / s->opts.tile_particles + 1;
     if (s->super_items && (hipMalloc(&s->item_bound, (size_t)s->max_items * sizeof(float2)) != hipSuccess ||
-                           hipMemsetAsync(s->item_bound, 0, (size_t)s->max_items * sizeof(float2), s->stream) != hipSuccess))
+                           hipMemsetAsync(s->item_bound, 0, (size_t)s->max_items * sizeof(float2), s->stream) != hipSuccess ||
+                           hipMalloc(&s->item_ref, (size_t)s->max_items * sizeof(float4)) != hipSuccess ||
+                           hipMemsetAsync(s->item_ref, 0, (size_t)s->max_items * sizeof(float4), s->stream) != hipSuccess ||
+                           hipMalloc(&s->item_shift, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
+                           hipMemsetAsync(s->item_shift, 0, (size_t)s->max_items * sizeof(int4), s->stream) != hipSuccess))
         return cleanup_fail(fail(MPM_E_HIP, "work-list allocation failed"));
     if (hipMalloc(&s->items, (size_t)s->max_items * sizeof(int4)) != hipSuccess ||
         hipMalloc(&s->n_items, sizeof(int)) != hipSuccess || hipMalloc(&s->d_bad, sizeof(unsigned long long)) != hipSuccess)
@@ -755,6 +761,8 @@ int mpm_destroy(mpm_solver* s)
     (void)hipFree(s->keys); (void)hipFree(s->vals); (void)hipFree(s->keys_alt); (void)hipFree(s->vals_alt);
     (void)hipFree(s->radix_scratch); (void)hipFree(s->items); (void)hipFree(s->n_items); (void)hipFree(s->d_bad);
     (void)hipFree(s->item_bound);
+    (void)hipFree(s->item_ref);
+    (void)hipFree(s->item_shift);
     (void)hipFree(s->scenes_d); (void)hipFree(s->block_scene_d); (void)hipFree(s->p_off_d);
     drop_graph(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);

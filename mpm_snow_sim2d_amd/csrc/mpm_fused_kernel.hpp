@@ -121,6 +121,13 @@ __device__ inline float wave_max(float v)
     return v;
 }
 
+__device__ inline float wave_sum(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 // What the scatter needs of one particle, kept in registers between the phases.
 template <int DIM>
 struct P2GIn {

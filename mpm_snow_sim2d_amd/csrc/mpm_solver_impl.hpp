@@ -73,6 +73,8 @@ struct mpm_solver {
     int nsx = 0, nsy = 0, nsz = 0;
     int key_bits_super = 0;          // bits of a super-block number (periodic re-binning)
     float2* item_bound = nullptr;
+    float4* item_ref = nullptr;      // mean particle position per item at binning
+    int4* item_shift = nullptr;      // tile offset per item (drift since binning)
     bool items_valid = false;
     bool sorted_keys_in_alt = false;
     int since_sort = 0;

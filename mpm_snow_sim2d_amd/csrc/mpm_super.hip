@@ -88,18 +88,24 @@ __device__ inline int fixed_cap(int cnt, float wmax)
 
 // Largest |m| and largest momentum bound of each item's particles in the stored
 // state (shaders/3d/p2g.glsl:44-59 stress), i.e. what a scatter of that state
-// would need: written after every binning and before a P2G-only pass.
+// would need, and their mean position (the reference of the tile's drift), with a
+// zero tile offset: written after every binning.
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork work, Phys ph)
 {
-    __shared__ float s_red[2][4];
+    __shared__ float s_red[5][4];
     const int n = P.n;
     const int nit = *work.n_items;
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int4 item = work.items[it];
-        float bm = 0.f, bpm = 0.f;
-        for (int q = threadIdx.x; q < item.z; q += 256) {
+        const int cnt = min(item.z, SB_MAX_PARTICLES);
+        int sc[3];
+        super_coords(item.x, work.nsx, work.nsy, sc);
+        float bm = 0.f, bpm = 0.f, msum[3] = {0.f, 0.f, 0.f};
+        for (int q = threadIdx.x; q < cnt; q += 256) {
             const int p = item.y + q;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) msum[d] += P.x[d * n + p] * ph.inv_dx - (float)(sc[d] * SB_CELLS);
             const float vol = P.vol[p];
             if (vol == 0.f) continue;                                // p2g.glsl:42
             const float pm = P.mass[p], Jp = P.Jp[p];
@@ -129,11 +135,25 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
         }
         bm = wave_max(bm);
         bpm = wave_max(bpm);
-        if ((threadIdx.x & 63) == 0) { s_red[0][threadIdx.x >> 6] = bm; s_red[1][threadIdx.x >> 6] = bpm; }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) msum[d] = wave_sum(msum[d]);
+        if ((threadIdx.x & 63) == 0) {
+            s_red[0][threadIdx.x >> 6] = bm;
+            s_red[1][threadIdx.x >> 6] = bpm;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s_red[2 + d][threadIdx.x >> 6] = msum[d];
+        }
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
             work.bound[it] = make_float2(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])),
                                          fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
+            const float inv = cnt > 0 ? 1.f / (float)cnt : 0.f;
+            float mean[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) mean[d] = ((s_red[2 + d][0] + s_red[2 + d][1]) + (s_red[2 + d][2] + s_red[2 + d][3])) * inv;
+            work.ref[it] = make_float4(mean[0], mean[1], mean[2], 0.f);
+            work.shift[it] = make_int4(0, 0, 0, 0);
+        }
         __syncthreads();
     }
 }
@@ -144,7 +164,9 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 {
     constexpr int T = SB_CELLS + 3 + 2 * MG;          // tile edge (nodes)
     constexpr int TN = T * T * T;
-    constexpr int NF = SB_BLOCKS + 2;                 // grid blocks a tile overlaps per axis
+    // grid blocks a tile overlaps per axis: ((o & 3) + T - 1) / 4 + 1 <= 4 for T <= 13
+    constexpr int NF = SB_BLOCKS + 2;
+    static_assert((3 + T - 1) / 4 + 1 <= NF, "tile overlaps more grid blocks than s_touch holds");
     constexpr int W = Spline<ORDER>::W;
     constexpr int NVR = COULOMB ? 6 : 3;
     // accumulators: per tile node (mass, p_x) and (p_y, p_z) as packed fixed point
@@ -153,7 +175,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
     // velocity in rows 3..5 with Coulomb walls
     __shared__ float s_vel[G2P ? NVR : 1][G2P ? TN : 1];
     __shared__ int s_touch[NF * NF * NF];
-    __shared__ float s_red[2][4];
+    __shared__ float s_red[5][4];
 
     const int n = P.n;
     const int tid = threadIdx.x;
@@ -163,13 +185,18 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         const int4 item = work.items[it];
         const int first = item.y, cnt = min(item.z, SB_MAX_PARTICLES);
         int sc[3];
-        sc[0] = item.x % work.nsx;
-        const int rest = item.x / work.nsx;
-        sc[1] = rest % work.nsy;
-        sc[2] = rest / work.nsy;
-        int o[3];
+        super_coords(item.x, work.nsx, work.nsy, sc);
+        // the tile follows the item's particles: its origin moves by the rounded mean
+        // drift since binning that the previous substep measured, so that a body
+        // moving as a whole stays in its tiles between binnings
+        const int4 shift = work.shift[it];
+        const int sh[3] = {shift.x, shift.y, shift.z};
+        int o[3], tb[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) o[a] = sc[a] * SB_CELLS - 1 - MG;
+        for (int a = 0; a < 3; ++a) {
+            o[a] = sc[a] * SB_CELLS - 1 - MG + sh[a];
+            tb[a] = o[a] >> 2;                        // first grid block the tile overlaps
+        }
 
         // ---- stage the tile (all loads issued before the first LDS store)
         int ltid = tid;
@@ -222,7 +249,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         const float scale_p = headroom_scale(prev.y + prev.x * (fabsf(ph.gravity) * ph.dt), cap);
         lds_barrier();
 
-        float bound_m = 0.f, bound_p = 0.f;
+        float bound_m = 0.f, bound_p = 0.f, msum[3] = {0.f, 0.f, 0.f};
         const int npass = (cnt + 63) >> 6;
         for (int pass = ((tid >> 6) + it) & 3; pass < npass; pass += 4) {
             const int q = pass * 64 + (tid & 63);
@@ -440,6 +467,8 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 kirchhoff<3>(ph, F, U, V, det<3>(F), Jp, tau);
             }
 
+#pragma unroll
+            for (int d = 0; d < 3; ++d) msum[d] += x[d] * ph.inv_dx - (float)(sc[d] * SB_CELLS);
             if (vol == 0.f) continue;                                 // p2g.glsl:42
             // ---- scatter (p2g.glsl): into the tile when the stencil lies in it and
             // the contributions fit the item's scale, else global float atomics
@@ -563,11 +592,29 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         // ---- this substep's bounds, for the next substep's scales
         bound_m = wave_max(bound_m);
         bound_p = wave_max(bound_p);
-        if ((tid & 63) == 0) { s_red[0][tid >> 6] = bound_m; s_red[1][tid >> 6] = bound_p; }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) msum[d] = wave_sum(msum[d]);
+        if ((tid & 63) == 0) {
+            s_red[0][tid >> 6] = bound_m;
+            s_red[1][tid >> 6] = bound_p;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s_red[2 + d][tid >> 6] = msum[d];
+        }
         lds_barrier();
-        if (tid == 0)
+        if (tid == 0) {
             work.bound[it] = make_float2(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])),
                                          fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
+            // the next substep's tile offset, from the mean of the positions this
+            // substep scattered from
+            const float4 ref = work.ref[it];
+            const float rf[3] = {ref.x, ref.y, ref.z};
+            const float inv = 1.f / (float)max(cnt, 1);
+            int ns[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                ns[d] = drift_shift(((s_red[2 + d][0] + s_red[2 + d][1]) + (s_red[2 + d][2] + s_red[2 + d][3])) * inv - rf[d]);
+            work.shift[it] = make_int4(ns[0], ns[1], ns[2], 0);
+        }
 
         // ---- flush: fixed point back to FP32, one global atomic per touched node
         // component, lanes over (node, component) pairs with the component fastest
@@ -590,18 +637,15 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
             float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, i, j, k)]) + c;
             unsafeAtomicAdd(dstp, (float)acc * (c == 0 ? inv_m : inv_p));
-            const int rbx = (i >> 2) - (sc[0] * SB_BLOCKS - 1);
-            const int rby = (j >> 2) - (sc[1] * SB_BLOCKS - 1);
-            const int rbz = (k >> 2) - (sc[2] * SB_BLOCKS - 1);
-            s_touch[(rbz * NF + rby) * NF + rbx] = 1;
+            s_touch[(((k >> 2) - tb[2]) * NF + ((j >> 2) - tb[1])) * NF + ((i >> 2) - tb[0])] = 1;
         }
         lds_barrier();
         int ftid = tid;
         asm volatile("" : "+v"(ftid));
         if (ftid < NF * NF * NF && s_touch[ftid]) {
-            const int bx = sc[0] * SB_BLOCKS - 1 + ftid % NF;
-            const int by = sc[1] * SB_BLOCKS - 1 + (ftid / NF) % NF;
-            const int bz = sc[2] * SB_BLOCKS - 1 + ftid / (NF * NF);
+            const int bx = tb[0] + ftid % NF;
+            const int by = tb[1] + (ftid / NF) % NF;
+            const int bz = tb[2] + ftid / (NF * NF);
             dst.flags[(bz * dst.nby + by) * dst.nbx + bx] = 1u;
         }
         lds_barrier();

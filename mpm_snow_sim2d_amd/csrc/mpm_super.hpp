@@ -19,8 +19,28 @@ struct SuperWork {
     const int*  n_items;
     int         max_items;
     float2*     bound;   // per item: largest |m| and largest momentum bound of its particles
+    float4*     ref;     // per item: its particles' mean position at binning, in cells from
+                         // the super-block's first cell
+    int4*       shift;   // per item: the tile's offset in cells, the rounded mean drift of
+                         // its particles since binning
     int         nsx, nsy;   // super-blocks along x and y
 };
+
+// Coordinates (in super-blocks) of super-block id sb.
+__host__ __device__ inline void super_coords(int sb, int nsx, int nsy, int* sc)
+{
+    sc[0] = sb % nsx;
+    const int rest = sb / nsx;
+    sc[1] = rest % nsy;
+    sc[2] = rest / nsy;
+}
+
+// Tile offset from a mean drift of d cells: its rounding, 0 where d is not finite or
+// beyond 64 cells (the particles are then far from any one tile).
+__host__ __device__ inline int drift_shift(float d)
+{
+    return fabsf(d) < 64.f ? (int)rintf(d) : 0;
+}
 
 // Super-blocks per axis for nb grid blocks along it.
 __host__ __device__ constexpr int super_count(int nb) { return (nb + SB_BLOCKS - 1) / SB_BLOCKS; }
