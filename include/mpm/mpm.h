@@ -62,10 +62,11 @@ typedef struct mpm_options {
     int fused;          /* 0: the four reference stages; 1: the fused G2P+P2G pipeline over work
                          * items of one grid block; 2 / 3: the fused pipeline over work items of
                          * 2x2x2 grid blocks (one 3D scene only; MPM_E_INVALID otherwise) whose
-                         * tile holds particles that drifted up to 0.5 / 1.5 cells since the
-                         * last binning (farther ones scatter with float atomics) */
+                         * tile follows the item's mean drift since the last binning and holds
+                         * particles up to 0.5 / 1.5 cells away from it (farther ones scatter
+                         * with float atomics) */
     int tile_particles; /* particles per work item at most: 1..512 for fused 1 (0 = 512),
-                         * 1..2048 for fused 2 and 3 (0 = 2048); longer runs split evenly     */
+                         * 1..4096 for fused 2 and 3 (0 = 4096); longer runs split evenly     */
 } mpm_options;
 
 /* Parameters.h defaults (src/shared/Parameters.h:30-84) for dim 2 or 3. */
@@ -187,7 +188,7 @@ int mpm_set_graph(mpm_solver* s, int enabled);
  * (rounded up to whole binning periods) on fresh solvers built from the same input,
  * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32, 64} with
  * fused = 1, and for one 3D scene also fused in {2, 3} x tile_particles in
- * {1024, 2048} x sort_every in {8, 16, 32, 64}, and returns the fastest options and
+ * {1024, 2048, 4096} x sort_every in {8, 16, 32, 64}, and returns the fastest options and
  * their time per substep.  nscenes = 1 tunes a single scene (params/init point to
  * one element).  The input is not modified. */
 int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,

@@ -24,7 +24,7 @@
 // 0.75^3 for the quadratic, (2/3)^3 for the cubic stencil); cap (fixed_cap) is the
 // largest for which cnt of them stay below 2^31, so each 32-bit half of a node's sum
 // cannot overflow, and every contribution is below 2^22 as the rounding trick of
-// fixed_bits needs.  For 512 to 2048 particles cap is 23 to 21, and the rounding of
+// fixed_bits needs.  For 512 to 4096 particles cap is 23 to 20, and the rounding of
 // a contribution is at most 2^-(cap+1) of the scale's power of two, 2^-cap of the
 // item's bound: the per-block kernel's resolution for the same item sizes.
 #include "mpm_fused_kernel.hpp"
