@@ -555,9 +555,9 @@ int create_impl(int nscenes, const mpm_params* params, const mpm_particles* init
     uint64_t maxkey = (uint64_t)s->nblocks * 64;
     int64_t nsuper = 0;
     if (s->super_items) {
-        s->nsx = super_count(s->nbx);
-        s->nsy = super_count(s->nby);
-        s->nsz = super_count(s->nbz);
+        s->nsx = super_count(s->params.WIDTH);
+        s->nsy = super_count(s->params.HEIGHT);
+        s->nsz = super_count(s->params.DEPTH);
         nsuper = (int64_t)s->nsx * s->nsy * s->nsz;
         maxkey = (uint64_t)nsuper << SB_KEY_SHIFT;
         if (maxkey > ((uint64_t)1 << 32)) return cleanup_fail(fail(MPM_E_INVALID, "grid too large for 32-bit super-block keys"));

@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         int o[3], tb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            o[a] = sc[a] * SB_CELLS - 1 - MG + sh[a];
+            o[a] = sc[a] * SB_CELLS - SB_CELLS - MG + sh[a];
             tb[a] = o[a] >> 2;                        // first grid block the tile overlaps
         }
 

@@ -42,19 +42,27 @@ __host__ __device__ inline int drift_shift(float d)
     return fabsf(d) < 64.f ? (int)rintf(d) : 0;
 }
 
-// Super-blocks per axis for nb grid blocks along it.
-__host__ __device__ constexpr int super_count(int nb) { return (nb + SB_BLOCKS - 1) / SB_BLOCKS; }
+// Super-block s along an axis holds cells [8 s - 7, 8 s]: the quadratic stencils of
+// its particles span nodes [8 s - 8, 8 s + 2], so its tile starts on a grid-block
+// boundary and each tile row covers 3 (not 4) blocks' 64-byte node runs.
+constexpr int SB_CELL_SHIFT = SB_CELLS - 1;
+
+// Super-block of cell c, and super-blocks per axis for cells 0..cells.
+__host__ __device__ constexpr int super_of_cell(int c) { return (c + SB_CELL_SHIFT) / SB_CELLS; }
+__host__ __device__ constexpr int super_count(int cells) { return super_of_cell(cells) + 1; }
 
 // Binning key of cell (cx, cy, cz) (inside the grid): super-block-major, then the
-// block within the super-block, then the cell within the block, x fastest; with
+// 4^3-cell quarter within the super-block, then the cell within that, x fastest; with
 // cells = false the super-block alone (key >> SB_KEY_SHIFT of the full key), so that a
 // stable sort keeps the previous order inside each super-block.
 __host__ __device__ inline uint32_t super_key(int cx, int cy, int cz, int nsx, int nsy, bool cells = true)
 {
-    const uint32_t sb = ((uint32_t)(cz >> 3) * (uint32_t)nsy + (uint32_t)(cy >> 3)) * (uint32_t)nsx + (uint32_t)(cx >> 3);
+    const uint32_t sb = ((uint32_t)super_of_cell(cz) * (uint32_t)nsy + (uint32_t)super_of_cell(cy)) * (uint32_t)nsx +
+                        (uint32_t)super_of_cell(cx);
     if (!cells) return sb;
-    const uint32_t lb = (((cz >> 2) & 1) << 2) | (((cy >> 2) & 1) << 1) | ((cx >> 2) & 1);
-    const uint32_t cell = ((cz & 3) << 4) | ((cy & 3) << 2) | (cx & 3);
+    const int lx = (cx + SB_CELL_SHIFT) & 7, ly = (cy + SB_CELL_SHIFT) & 7, lz = (cz + SB_CELL_SHIFT) & 7;
+    const uint32_t lb = ((lz >> 2) << 2) | ((ly >> 2) << 1) | (lx >> 2);
+    const uint32_t cell = ((lz & 3) << 4) | ((ly & 3) << 2) | (lx & 3);
     return (sb << SB_KEY_SHIFT) | (lb << 6) | cell;
 }
 

@@ -93,17 +93,18 @@ def cell_keys(params: Params, x: np.ndarray) -> np.ndarray:
 def super_keys(params: Params, x: np.ndarray) -> np.ndarray:
     """CPU restatement of the 3D super-block binning key (mpm_options.fused 2 or 3):
     the cell that holds the particle (clamped into the grid), numbered
-    super-block-major (2x2x2 grid blocks of 4^3 cells), then by the block within the
-    super-block, then by the cell within the block, x fastest."""
+    super-block-major (super-block s along an axis holds cells 8s-7 .. 8s, so that
+    its particles' stencils start on a grid-block boundary), then by the 4^3-cell
+    quarter within the super-block, then by the cell within that, x fastest."""
     inv = np.float32(1.0) / np.float32(params.dx)
     lim = [params.WIDTH, params.HEIGHT, params.DEPTH]
     c = [np.clip(np.floor(np.asarray(x[a], np.float32) * inv).astype(np.int64), 0, lim[a]) for a in range(3)]
-    nbx = (params.WIDTH + 1 + 3) // 4
-    nby = (params.HEIGHT + 1 + 3) // 4
-    nsx, nsy = (nbx + 1) // 2, (nby + 1) // 2
-    sb = ((c[2] >> 3) * nsy + (c[1] >> 3)) * nsx + (c[0] >> 3)
-    lb = (((c[2] >> 2) & 1) << 2) | (((c[1] >> 2) & 1) << 1) | ((c[0] >> 2) & 1)
-    cell = ((c[2] & 3) << 4) | ((c[1] & 3) << 2) | (c[0] & 3)
+    nsx, nsy = (params.WIDTH + 7) // 8 + 1, (params.HEIGHT + 7) // 8 + 1
+    s = [(ca + 7) >> 3 for ca in c]
+    loc = [(ca + 7) & 7 for ca in c]
+    sb = (s[2] * nsy + s[1]) * nsx + s[0]
+    lb = ((loc[2] >> 2) << 2) | ((loc[1] >> 2) << 1) | (loc[0] >> 2)
+    cell = ((loc[2] & 3) << 4) | ((loc[1] & 3) << 2) | (loc[0] & 3)
     return ((sb << 9) | (lb << 6) | cell).astype(np.uint32)
 
 
