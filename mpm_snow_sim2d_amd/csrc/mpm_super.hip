@@ -505,9 +505,14 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 }
             if (tile_ok && inside) {
                 // whole stencil in the tile and the grid: scaled inputs, one rounding
-                // per component, two packed 64-bit LDS adds per node; each lane walks
-                // its stencil from a start rotated by its lane index, so that sorted
-                // neighbours with one stencil base add to different words at a time
+                // per component, two packed 64-bit LDS adds per node.  Each lane walks
+                // its stencil from a rotated start: at step t it adds to node
+                // b + ((t + rho) mod W) per axis, whose residue mod W, b + rho + t, moves
+                // in step for every lane; rho is chosen so that b + rho mod W is the
+                // lane's own phase (its index's base-W digits), and lanes of different
+                // phases then never add to the same word at the same step, whatever
+                // their bases (27 phases for the quadratic stencil: lanes 27 apart share
+                // one)
                 const float pms = pm * scale_m;
                 float mvs[3], As[9];
 #pragma unroll
@@ -524,8 +529,10 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                     asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        const int rot = lr % W;
+                        const int phase = lr % W;
                         lr /= W;
+                        int rot = phase - rb[d] % W;       // rb[d] >= 0 in the tile
+                        rot = rot < 0 ? rot + W : rot;
 #pragma unroll
                         for (int s2 = 0; s2 < W; ++s2) {
                             const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
