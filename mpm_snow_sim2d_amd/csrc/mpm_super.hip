@@ -632,19 +632,33 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 #pragma unroll
             for (int a = 0; a < 3; ++a) tile_in &= o[a] >= 0 && o[a] + T - 1 <= lim3[a];
         }
-        for (int pe = tid; pe < TN * 4; pe += 256) {
-            const int e = pe >> 2, c = pe & 3;
-            const unsigned long long sum = s_acc[c >> 1][e];
-            const int acc = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
-            // a node whose mass rounded to zero flushes no momentum either: mass and
-            // momentum round independently, and a momentum unit over a float-path
-            // mass far below the mass unit would give the node any velocity
-            if (acc == 0 || (c != 0 && unpack_lo(s_acc[0][e]) == 0)) continue;
-            const int i = o[0] + e % T, j = o[1] + (e / T) % T, k = o[2] + e / (T * T);
-            if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
-            float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, i, j, k)]) + c;
-            unsafeAtomicAdd(dstp, (float)acc * (c == 0 ? inv_m : inv_p));
-            s_touch[(((k >> 2) - tb[2]) * NF + ((j >> 2) - tb[1])) * NF + ((i >> 2) - tb[0])] = 1;
+        // four passes' LDS words read together, so that their latencies overlap
+        constexpr int FU = 4;
+        const int c = tid & 3;
+        for (int pe0 = tid; pe0 < TN * 4; pe0 += 256 * FU) {
+            int acc[FU], mass[FU];
+#pragma unroll
+            for (int u = 0; u < FU; ++u) {
+                const int e = min(pe0 + 256 * u, TN * 4 - 1) >> 2;
+                const unsigned long long sum = s_acc[c >> 1][e];
+                acc[u] = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
+                mass[u] = unpack_lo(s_acc[0][e]);
+            }
+#pragma unroll
+            for (int u = 0; u < FU; ++u) {
+                const int pe = pe0 + 256 * u;
+                // a node whose mass rounded to zero flushes no momentum either: mass
+                // and momentum round independently, and a momentum unit over a
+                // float-path mass far below the mass unit would give the node any
+                // velocity
+                if (pe >= TN * 4 || acc[u] == 0 || mass[u] == 0) continue;
+                const int e = pe >> 2;
+                const int i = o[0] + e % T, j = o[1] + (e / T) % T, k = o[2] + e / (T * T);
+                if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
+                float* dstp = reinterpret_cast<float*>(&dst.a[node_addr<3>(dst, i, j, k)]) + c;
+                unsafeAtomicAdd(dstp, (float)acc[u] * (c == 0 ? inv_m : inv_p));
+                s_touch[(((k >> 2) - tb[2]) * NF + ((j >> 2) - tb[1])) * NF + ((i >> 2) - tb[0])] = 1;
+            }
         }
         lds_barrier();
         int ftid = tid;
