@@ -5,7 +5,9 @@ calls with host work and synchronisation between them.  The capture adds its
 workgroups' boxes into the image with float atomics, so two step-by-step runs
 already differ at round-off (and the network's BF16 inputs can round either way
 where they do): the replayed rollout may differ from one of them by at most 4x what
-the two differ by, plus a floor at FP32 resolution.  A failing residual test stops
+the two differ by, plus a floor of ~80 FP32 ulps at 1 (the largest difference over
+~3000 particles and 12 macro steps is a single particle's, so the run-to-run spread
+alone can exceed the factor).  A failing residual test stops
 the rollout at that macro step with the particles as they were before it."""
 import numpy as np
 import pytest
@@ -30,7 +32,7 @@ def test_graph_rollout_matches_stepwise_advance(mode_name):
     k = 8
     _flush(f"[rollout] graph vs stepwise, {mode_name}, n={ps.n}")
     from tests.helpers import state_errors
-    floor = {"x": 1e-4, "v": 1e-6, "C": 1e-6, "F": 1e-6, "Jp": 1e-6}
+    floor = {"x": 1e-4, "v": 1e-5, "C": 1e-5, "F": 1e-5, "Jp": 1e-5}
 
     def check(sa, sb, sc):
         e_ab = state_errors(sa, sb, float(P.dx))
