@@ -154,6 +154,10 @@ int mpm_set_timing(mpm_solver* s, int enabled);
  * event records between launches when one kernel is measured.  No reference
  * counterpart (instrumentation of this library's own kernels). */
 int mpm_set_timing_mask(mpm_solver* s, uint32_t kernels);
+/* Time only every k-th launch of each timed kernel (1, the default: every launch),
+ * counting from this call: fewer event records between launches, the same average
+ * per launch.  No reference counterpart. */
+int mpm_set_timing_every(mpm_solver* s, int every);
 int mpm_reset_timers(mpm_solver* s);
 int mpm_get_kernel_time(mpm_solver* s, int kernel, double* total_ms, int64_t* launches);
 

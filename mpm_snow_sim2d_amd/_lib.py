@@ -30,7 +30,7 @@ EXPORTS = [
     "mpm_sort", "mpm_get_sort", "mpm_sync", "mpm_get_positions",
     "mpm_get_state", "mpm_set_state", "mpm_get_grid", "mpm_get_grid_force",
     "mpm_num_particles", "mpm_num_nodes",
-    "mpm_set_timing", "mpm_set_timing_mask", "mpm_reset_timers", "mpm_get_kernel_time",
+    "mpm_set_timing", "mpm_set_timing_mask", "mpm_set_timing_every", "mpm_reset_timers", "mpm_get_kernel_time",
     "mpm_check_finite", "mpm_sort_pairs_host", "mpm_sort_pairs_host_noalloc", "mpm_last_error",
     "mpm_batch_create", "mpm_num_scenes", "mpm_scene_range", "mpm_set_graph", "mpm_autotune",
     "mpm_surrogate_create", "mpm_surrogate_destroy", "mpm_surrogate_num_params", "mpm_surrogate_get_params",
@@ -102,6 +102,7 @@ def load() -> ctypes.CDLL:
         "mpm_num_nodes": (i32, [P(Params), P(i64)]),
         "mpm_set_timing": (i32, [vp, i32]),
         "mpm_set_timing_mask": (i32, [vp, ctypes.c_uint32]),
+        "mpm_set_timing_every": (i32, [vp, i32]),
         "mpm_reset_timers": (i32, [vp]),
         "mpm_get_kernel_time": (i32, [vp, i32, P(ctypes.c_double), P(i64)]),
         "mpm_check_finite": (i32, [vp, P(i64)]),

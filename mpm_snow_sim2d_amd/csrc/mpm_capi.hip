@@ -50,7 +50,7 @@ template <typename Fn>
 int timed(mpm_solver* s, int kernel, Fn&& fn)
 {
     hipEvent_t a = nullptr, b = nullptr;
-    const bool on = s->timing && ((s->timing_mask >> kernel) & 1u);
+    const bool on = s->timing && ((s->timing_mask >> kernel) & 1u) && s->timing_seen[kernel]++ % s->timing_every == 0;
     if (on) {
         a = s->take_event();
         b = s->take_event();
@@ -1026,6 +1026,15 @@ int mpm_set_timing_mask(mpm_solver* s, uint32_t kernels)
     if (!s) return fail(MPM_E_INVALID, "solver is NULL");
     s->timing = kernels != 0;
     s->timing_mask = kernels;
+    return MPM_OK;
+}
+
+int mpm_set_timing_every(mpm_solver* s, int every)
+{
+    if (!s) return fail(MPM_E_INVALID, "solver is NULL");
+    if (every < 1) return fail(MPM_E_INVALID, "every must be >= 1");
+    s->timing_every = every;
+    for (auto& c : s->timing_seen) c = 0;
     return MPM_OK;
 }
 

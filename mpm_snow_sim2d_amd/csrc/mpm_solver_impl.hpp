@@ -102,6 +102,8 @@ struct mpm_solver {
 
     bool timing = false;
     uint32_t timing_mask = ~0u;   // kernels timed while timing is on (bit MPM_KERNEL_*)
+    int timing_every = 1;           // time every k-th launch of each timed kernel
+    int64_t timing_seen[MPM_KERNEL_COUNT] = {};   // launches of each kernel while timed
     mpm::detail::KernelTimer timers[MPM_KERNEL_COUNT];
     std::vector<hipEvent_t> event_pool;
 

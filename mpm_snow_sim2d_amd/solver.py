@@ -173,6 +173,10 @@ class GPUSolver:
                 mask |= 1 << int(k)
             check(self._lib.mpm_set_timing_mask(self._h, mask if on else 0))
 
+    def set_timing_every(self, every: int) -> None:
+        """Time only every k-th launch of each timed kernel (mpm_set_timing_every)."""
+        check(self._lib.mpm_set_timing_every(self._h, int(every)))
+
     def reset_timers(self) -> None:
         check(self._lib.mpm_reset_timers(self._h))
 
@@ -349,6 +353,10 @@ class BatchSolver:
             for k in kernels:
                 mask |= 1 << int(k)
             check(self._lib.mpm_set_timing_mask(self._h, mask if on else 0))
+
+    def set_timing_every(self, every: int) -> None:
+        """Time only every k-th launch of each timed kernel (mpm_set_timing_every)."""
+        check(self._lib.mpm_set_timing_every(self._h, int(every)))
 
     def reset_timers(self) -> None:
         check(self._lib.mpm_reset_timers(self._h))
