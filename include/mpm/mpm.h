@@ -193,7 +193,7 @@ int mpm_set_graph(mpm_solver* s, int enabled);
  * (rounded up to whole binning periods) on fresh solvers built from the same input,
  * over tile_particles in {128, 256, 512} x sort_every in {4, 8, 16, 32, 64} with
  * fused = 1, and for one 3D scene also fused in {2, 3} x tile_particles in
- * {1024, 2048, 4096} x sort_every in {8, 16, 32, 64}, and returns the fastest options and
+ * {1024, 2048, 4096} x sort_every in {8, 16, 32, 64, 128}, and returns the fastest options and
  * their time per substep.  nscenes = 1 tunes a single scene (params/init point to
  * one element).  The input is not modified. */
 int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* init, int trial_substeps,

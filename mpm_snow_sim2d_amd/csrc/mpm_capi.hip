@@ -706,7 +706,7 @@ int mpm_autotune(int nscenes, const mpm_params* params, const mpm_particles* ini
     if (nscenes == 1 && params && params[0].dim == 3)
         for (int f : {2, 3})
             for (int tp : {SB_MAX_PARTICLES / 4, SB_MAX_PARTICLES / 2, SB_MAX_PARTICLES})
-                for (int se : {8, 16, 32, 64}) cands.push_back(mpm_options{se, f, tp});
+                for (int se : {8, 16, 32, 64, 128}) cands.push_back(mpm_options{se, f, tp});
     double best_ms = -1.0;
     mpm_options chosen{};
     for (const mpm_options& o : cands) {
