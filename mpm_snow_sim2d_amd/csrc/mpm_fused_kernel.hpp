@@ -621,9 +621,9 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 // the item's power-of-two scales (exact), one rounding per component
                 // (fixed_bits) and LDS adds (3D: two of packed component pairs per
                 // node).  In 3D each lane walks its stencil from a start rotated by its
-                // lane index, so that lanes whose particles share a stencil base
-                // (sorted neighbours: a 3D pass holds consecutive particles) add to
-                // different words at each step instead of serialising on one; 2D
+                // phase, so that lanes whose particles share stencil nodes (sorted
+                // neighbours: a 3D pass holds consecutive particles) add to different
+                // words at each step instead of serialising on one; 2D
                 // passes spread their lanes over the block's cells already, and the
                 // rotation measured slower there (81.5 against 88.3 us on the batch).
                 const float pms = in[r].pm * scale_m;
@@ -645,8 +645,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                     asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        const int rot = lr % W;
+                        // 3D: the lane's phase (its index's base-W digits) fixes
+                        // (base + rot) mod W, so that lanes of different phases never
+                        // add to one word at the same step (as in k_super)
+                        const int phase = lr % W;
                         lr /= W;
+                        int rot = DIM == 3 ? phase - rb[d] % W : phase;
+                        rot = rot < 0 ? rot + W : rot;
 #pragma unroll
                         for (int s2 = 0; s2 < W; ++s2) {
                             const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
