@@ -620,12 +620,13 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 // through the tile, whole stencil inside: the inputs pre-multiplied by
                 // the item's power-of-two scales (exact), one rounding per component
                 // (fixed_bits) and LDS adds (3D: two of packed component pairs per
-                // node).  In 3D each lane walks its stencil from a start rotated by its
-                // phase, so that lanes whose particles share stencil nodes (sorted
-                // neighbours: a 3D pass holds consecutive particles) add to different
-                // words at each step instead of serialising on one; 2D
-                // passes spread their lanes over the block's cells already, and the
-                // rotation measured slower there (81.5 against 88.3 us on the batch).
+                // node).  Each lane walks its stencil from a start rotated by its
+                // phase, so that lanes whose particles share stencil nodes (3D: sorted
+                // neighbours, a pass holds consecutive particles; 2D: a pass's lanes
+                // spread over the block's cells) add to different words at each step
+                // instead of serialising on one (2D batch: 75.9 against 69.1 us; a
+                // rotation by the lane's digits alone measured slower there, 88.3
+                // against 81.5 us).
                 const float pms = in[r].pm * scale_m;
                 float mvs[DIM], As[DIM * DIM];
 #pragma unroll
@@ -641,16 +642,16 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                     // the lane's rotation, from an opaque copy of its index: derived per
                     // particle (a few integer operations) instead of being hoisted out
                     // of the item loop as compare masks kept live (or spilled) across G2P
-                    int lr = DIM == 3 ? tid & 63 : 0;
+                    int lr = tid & 63;
                     asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        // 3D: the lane's phase (its index's base-W digits) fixes
+                        // the lane's phase (its index's base-W digits) fixes
                         // (base + rot) mod W, so that lanes of different phases never
                         // add to one word at the same step (as in k_super)
                         const int phase = lr % W;
                         lr /= W;
-                        int rot = DIM == 3 ? phase - rb[d] % W : phase;
+                        int rot = phase - rb[d] % W;
                         rot = rot < 0 ? rot + W : rot;
 #pragma unroll
                         for (int s2 = 0; s2 < W; ++s2) {
