@@ -234,8 +234,10 @@ TileWork tile_work(const mpm_solver* s)
     return w;
 }
 
-// One fused substep: binning when due, grid update + clear, fused G2P/P2G.
-int fused_substep(mpm_solver* s)
+// One fused substep: binning when due, grid update + clear, fused G2P/P2G.  With
+// super-block items, last: the substep ends the caller's step (it stores v and C,
+// which the substeps between binnings leave to the next gather).
+int fused_substep(mpm_solver* s, bool last)
 {
     int r;
     if (s->opts.sort_every > 0 && s->since_sort >= s->opts.sort_every && (r = do_sort(s))) return r;
@@ -249,8 +251,11 @@ int fused_substep(mpm_solver* s)
     DevParticles p = s->dev_particles();
     if (s->super_items) {
         const SuperWork w = super_work(s);
+        // v and C are read after this substep by the next binning's item bounds
+        // (due when the next substep starts) or by the caller
+        const bool store_vc = last || (s->opts.sort_every > 0 && s->since_sort + 1 >= s->opts.sort_every);
         r = timed(s, MPM_KERNEL_G2P2G, [&] {
-            return launch_super(s->cfg, s->super_margin, true, p, cur, old, w, s->ph, s->stream);
+            return launch_super(s->cfg, s->super_margin, true, store_vc, p, cur, old, w, s->ph, s->stream);
         });
     } else {
         TileWork w = tile_work(s);
@@ -284,7 +289,7 @@ int run_graph_period(mpm_solver* s, int period)
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         const int grid0 = s->cur_grid, pb0 = s->cur_pb;
         int r = MPM_OK;
-        for (int k = 0; k < period && r == MPM_OK; ++k) r = fused_substep(s);
+        for (int k = 0; k < period && r == MPM_OK; ++k) r = fused_substep(s, k == period - 1);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &graph);
         s->timing = timing;
@@ -315,7 +320,8 @@ int step_fused(mpm_solver* s, int nsub)
             const SuperWork w = super_work(s);
             r = timed(s, MPM_KERNEL_P2G, [&] {
                 hipError_t e = launch_item_bounds(s->cfg.order, p, w, s->ph, s->stream);
-                return e != hipSuccess ? e : launch_super(s->cfg, s->super_margin, false, p, g, g, w, s->ph, s->stream);
+                return e != hipSuccess ? e
+                                       : launch_super(s->cfg, s->super_margin, false, true, p, g, g, w, s->ph, s->stream);
             });
         } else {
             TileWork w = tile_work(s);
@@ -334,7 +340,7 @@ int step_fused(mpm_solver* s, int nsub)
             k += period;
             continue;
         }
-        if ((r = fused_substep(s))) return r;
+        if ((r = fused_substep(s, k == nsub - 1))) return r;
         ++k;
     }
     return MPM_OK;

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
 
 template <int ORDER, bool COULOMB, bool G2P, int MG>
 __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, DevGrid dst, SuperWork work,
-                                                  Phys ph)
+                                                  Phys ph, bool store_vc)
 {
     constexpr int T = SB_CELLS + 3 + 2 * MG;          // tile edge (nodes)
     constexpr int TN = T * T * T;
@@ -451,11 +451,19 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 }
 #pragma unroll
                 for (int d = 0; d < 3; ++d) { x[d] = nx[d]; v[d] = nv[d]; }
+                // v and C only when a reader follows (the next substep's gather
+                // recomputes both from the grid)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) { P.x[d * n + p] = x[d]; P.v[d * n + p] = v[d]; }
+                for (int d = 0; d < 3; ++d) P.x[d * n + p] = x[d];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) { P.C[e * n + p] = C[e]; P.F[e * n + p] = F[e]; }
+                for (int e = 0; e < 9; ++e) P.F[e * n + p] = F[e];
                 P.Jp[p] = Jp;
+                if (store_vc) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) P.v[d * n + p] = v[d];
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) P.C[e * n + p] = C[e];
+                }
             } else {
                 // stress from the stored state (p2g.glsl:44-59)
                 float F[9], U[9], sig[3], V[9];
@@ -714,8 +722,8 @@ hipError_t launch_item_bounds(int order, const DevParticles& p, const SuperWork&
     return hipGetLastError();
 }
 
-hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, const DevParticles& p, const DevGrid& src,
-                        const DevGrid& dst, const SuperWork& w, const Phys& ph, hipStream_t st)
+hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, bool store_vc, const DevParticles& p,
+                        const DevGrid& src, const DevGrid& dst, const SuperWork& w, const Phys& ph, hipStream_t st)
 {
     if (p.n == 0) return hipSuccess;
     if (c.dim != 3) return hipErrorInvalidValue;
@@ -723,7 +731,7 @@ hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, const DevParti
     // the P2G-only pass gathers nothing, so the wall kind does not enter it
     const bool coul = g2p && c.boundary == MPM_BOUNDARY_COULOMB_MARGIN;
 #define SCASE(O, CL, G, M) \
-    if (c.order == O && coul == CL && g2p == G && margin == M) { k_super<O, CL, G, M><<<grid, blk, 0, st>>>(p, src, dst, w, ph); return hipGetLastError(); }
+    if (c.order == O && coul == CL && g2p == G && margin == M) { k_super<O, CL, G, M><<<grid, blk, 0, st>>>(p, src, dst, w, ph, store_vc); return hipGetLastError(); }
     SCASE(2, false, true, 0) SCASE(2, false, true, 1) SCASE(2, true, true, 0) SCASE(2, true, true, 1)
     SCASE(2, false, false, 0) SCASE(2, false, false, 1)
     SCASE(3, false, true, 0) SCASE(3, false, true, 1) SCASE(3, true, true, 0) SCASE(3, true, true, 1)

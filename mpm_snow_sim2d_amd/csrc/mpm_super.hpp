@@ -73,7 +73,11 @@ hipError_t launch_item_bounds(int order, const DevParticles& p, const SuperWork&
                               hipStream_t st);
 // g2p: one fused substep (G2P from src, P2G into dst); otherwise the P2G of the
 // stored state into dst.  margin: drift margin of the tile in cells (0 or 1).
-hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, const DevParticles& p, const DevGrid& src,
-                        const DevGrid& dst, const SuperWork& w, const Phys& ph, hipStream_t st);
+// store_vc: the substep also stores each particle's v and C (x, F_E and J_P always);
+// the next fused substep recomputes v and C by its gather, so only a substep that a
+// reader of them follows (binning's item bounds, the caller) needs to.
+hipError_t launch_super(const LaunchCfg& c, int margin, bool g2p, bool store_vc, const DevParticles& p,
+                        const DevGrid& src, const DevGrid& dst, const SuperWork& w, const Phys& ph,
+                        hipStream_t st);
 
 }  // namespace mpm

@@ -160,6 +160,47 @@ def test_super_after_stage_calls_and_state_changes(orc):
         assert err[k][0] < 1e-3, (k, err[k])
 
 
+def test_super_step_ends_with_its_last_substeps_v_and_c():
+    """The fused substeps between binnings leave v and C unstored (the next gather
+    recomputes both); the substep that ends a step, or that a binning follows,
+    stores them.  A step split into calls of 3, 4 and 2 substeps (binning every 4)
+    ends with the state of 9 substeps in one call, and so does a graph-replayed run
+    (periods of 8) against an eager one, each to 1e-5 in natural units (x: 1e-4
+    cells, C: 4e-6; over the 20 replayed substeps x 1e-3 cells, v and C 2e-5),
+    while one substep fewer already differs in v by more than 5e-5 (v and C are
+    stored together, so the check would see them left one substep stale)."""
+    from mpm_snow_sim2d_amd.solver import GPUSolver
+    P, ps = make_case("snow3d_quad_sticky")
+    opts = Options(fused=2, sort_every=4, tile_particles=0)
+    _flush(f"[super] v/C at step ends, n={ps.n}")
+    with GPUSolver(P, ps, opts) as a, GPUSolver(P, ps, opts) as b, GPUSolver(P, ps, opts) as c:
+        a.Step(9)
+        for k in (3, 4, 2):
+            b.Step(k)
+        c.Step(8)
+        sa, sb, sc = a.get_state(), b.get_state(), c.get_state()
+    e_split = state_errors(sb, sa, P.dx)
+    e_stale = state_errors(sc, sa, P.dx)
+    _flush(f"    split vs whole {e_split}\n    8 vs 9 substeps {e_stale}")
+    tol = {"x": 1e-4, "v": 1e-5, "C": 4e-6, "F": 1e-5, "Jp": 1e-5}
+    for k in tol:
+        assert e_split[k][0] <= tol[k], (k, e_split[k])
+    # v and C are stored together: a stale v shows a skipped store
+    assert e_stale["v"][0] > 5 * tol["v"], e_stale
+    with GPUSolver(P, ps, opts) as g, GPUSolver(P, ps, opts) as e:
+        g.set_graph(True)
+        g.Step(20)
+        e.Step(20)
+        sg, se = g.get_state(), e.get_state()
+    e_graph = state_errors(sg, se, P.dx)
+    _flush(f"    graph vs eager {e_graph}")
+    # over 20 substeps the runs' float-atomic orders spread x, v and C further
+    # (~1.5e-4 cells, ~2.5e-6, ~5e-6); v keeps 8x the power to see a stale store
+    tol_g = dict(tol, x=1e-3, v=2e-5, C=2e-5)
+    for k in tol_g:
+        assert e_graph[k][0] <= tol_g[k], (k, e_graph[k])
+
+
 def test_super_refuses_2d_and_batches():
     from mpm_snow_sim2d_amd.solver import BatchSolver, GPUSolver, MPMError
     P2, ps2 = make_case("snow2d_quad_sticky")
