@@ -231,12 +231,14 @@ TileWork tile_work(const mpm_solver* s)
     w.items = s->items;
     w.n_items = s->n_items;
     w.max_items = s->max_items;
+    w.store_vc = 1;
     return w;
 }
 
-// One fused substep: binning when due, grid update + clear, fused G2P/P2G.  With
-// super-block items, last: the substep ends the caller's step (it stores v and C,
-// which the substeps between binnings leave to the next gather).
+// One fused substep: binning when due, grid update + clear, fused G2P/P2G.  last:
+// the substep ends the caller's step (it stores v and C, which the other substeps
+// leave to the next gather; with super-block items also the substep before a
+// binning, whose item bounds read them).
 int fused_substep(mpm_solver* s, bool last)
 {
     int r;
@@ -259,6 +261,7 @@ int fused_substep(mpm_solver* s, bool last)
         });
     } else {
         TileWork w = tile_work(s);
+        w.store_vc = last ? 1 : 0;
         r = timed(s, MPM_KERNEL_G2P2G,
                   [&] { return launch_g2p2g(s->cfg, p, cur, old, w, s->ph, s->scenes_d, s->stream); });
     }

@@ -511,11 +511,19 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                 Jp = Jp * ratio;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) { x[d] = nx[d]; v[d] = nv[d]; }
+                // v and C only when a reader follows (the next substep's gather
+                // recomputes both from the grid)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) { P.x[d * n + p] = x[d]; P.v[d * n + p] = v[d]; }
+                for (int d = 0; d < DIM; ++d) P.x[d * n + p] = x[d];
 #pragma unroll
-                for (int e = 0; e < DIM * DIM; ++e) { P.C[e * n + p] = C[e]; P.F[e * n + p] = F[e]; }
+                for (int e = 0; e < DIM * DIM; ++e) P.F[e * n + p] = F[e];
                 P.Jp[p] = Jp;
+                if (work.store_vc) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) P.v[d * n + p] = v[d];
+#pragma unroll
+                    for (int e = 0; e < DIM * DIM; ++e) P.C[e * n + p] = C[e];
+                }
                 // stress of the new state: U diag(2 mu (s-1) s + la (J-1) J) U^T
                 float harden = expf(ph.xi * (1.f - Jp));
                 float tmu = 2.f * (ph.mu * harden);

@@ -33,6 +33,9 @@ struct TileWork {
     const int4* items;   // (block id, first particle, count, 0) per work item
     const int*  n_items;
     int         max_items;
+    int         store_vc;   // the substep stores v and C (x, F_E, J_P always): the next
+                            // fused substep's gather recomputes both, so only a substep
+                            // whose state the caller reads needs to
 };
 // scenes == nullptr: one scene with parameters ph; otherwise a batch whose items
 // carry their scene index (item.w) into the scene table.
