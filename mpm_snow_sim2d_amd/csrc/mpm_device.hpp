@@ -397,8 +397,10 @@ __host__ __device__ inline void elastic_bounds(float theta0, float theta1, float
     *hi2 = fminf(theta1 * theta1, 1.1f) * (1.f - 9.5367431640625e-7f);
 }
 
-// True when F (row-major 3x3) passes the test; then *vmi = V - I and *J = det F.
-__host__ __device__ inline bool elastic_stretch(const float* F, float lo2, float hi2, Sym3* vmi, float* J)
+// True when F (row-major 3x3) passes the test; then *vmi = V - I, *bmv = B - V =
+// (V - I) V (what the stress needs: V^2 = B) and *J = det F.
+__host__ __device__ inline bool elastic_stretch(const float* F, float lo2, float hi2, Sym3* vmi, Sym3* bmv,
+                                                float* J)
 {
     Sym3 B;
     B.xx = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
@@ -418,22 +420,25 @@ __host__ __device__ inline bool elastic_stretch(const float* F, float lo2, float
     P = sym_add_diag(sym_mul(E, P), 1.f / 16.f);
     P = sym_add_diag(sym_mul(E, P), -1.f / 8.f);
     P = sym_add_diag(sym_mul(E, P), 0.5f);
-    *vmi = sym_mul(E, P);
+    const Sym3 v = sym_mul(E, P);
+    *vmi = v;
+    bmv->xx = E.xx - v.xx; bmv->yy = E.yy - v.yy; bmv->zz = E.zz - v.zz;
+    bmv->xy = E.xy - v.xy; bmv->xz = E.xz - v.xz; bmv->yz = E.yz - v.yz;
     return true;
 }
 
 // Kirchhoff stress of an elastic state from its left stretch: 2 mu h (V - I) V +
-// lambda h (J - 1) J I with (V - I) V = (V - I) + (V - I)^2, h = exp(xi (1 - J_P)).
-__host__ __device__ inline void elastic_kirchhoff(const Sym3& vmi, float J, float tmu, float la, float* tau)
+// lambda h (J - 1) J I with (V - I) V = B - V (bmv from elastic_stretch), h =
+// exp(xi (1 - J_P)).
+__host__ __device__ inline void elastic_kirchhoff(const Sym3& bmv, float J, float tmu, float la, float* tau)
 {
-    const Sym3 q = sym_mul(vmi, vmi);
     const float sI = la * (J - 1.f) * J;
-    tau[0] = tmu * (vmi.xx + q.xx) + sI;
-    tau[4] = tmu * (vmi.yy + q.yy) + sI;
-    tau[8] = tmu * (vmi.zz + q.zz) + sI;
-    tau[1] = tau[3] = tmu * (vmi.xy + q.xy);
-    tau[2] = tau[6] = tmu * (vmi.xz + q.xz);
-    tau[5] = tau[7] = tmu * (vmi.yz + q.yz);
+    tau[0] = tmu * bmv.xx + sI;
+    tau[4] = tmu * bmv.yy + sI;
+    tau[8] = tmu * bmv.zz + sI;
+    tau[1] = tau[3] = tmu * bmv.xy;
+    tau[2] = tau[6] = tmu * bmv.xz;
+    tau[5] = tau[7] = tmu * bmv.yz;
 }
 
 // Kirchhoff stress of F with polar rotation R = U V^T (shaders/2d/p2g.glsl:44-59):

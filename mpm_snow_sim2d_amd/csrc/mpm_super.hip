@@ -118,10 +118,10 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
             // kernel takes it, otherwise through the SVD
             float elo2, ehi2, J;
             elastic_bounds(ph.theta0, ph.theta1, &elo2, &ehi2);
-            Sym3 vmi;
-            if (elastic_stretch(F, elo2, ehi2, &vmi, &J)) {
+            Sym3 vmi, bmv;
+            if (elastic_stretch(F, elo2, ehi2, &vmi, &bmv, &J)) {
                 const float harden = expf(ph.xi * (1.f - Jp));
-                elastic_kirchhoff(vmi, J, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
+                elastic_kirchhoff(bmv, J, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
             } else {
                 float U[9], sig[3], V[9];
                 svd3(F, U, sig, V);
@@ -404,12 +404,12 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 // skips the SVD: F_E' = F^, J_P' = J_P, stress from the left stretch
                 float elo2, ehi2, Jh;
                 elastic_bounds(ph.theta0, ph.theta1, &elo2, &ehi2);
-                Sym3 vmi;
-                if (elastic_stretch(Fh, elo2, ehi2, &vmi, &Jh)) {
+                Sym3 vmi, bmv;
+                if (elastic_stretch(Fh, elo2, ehi2, &vmi, &bmv, &Jh)) {
 #pragma unroll
                     for (int e = 0; e < 9; ++e) F[e] = Fh[e];
                     const float harden = expf(ph.xi * (1.f - Jp));
-                    elastic_kirchhoff(vmi, Jh, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
+                    elastic_kirchhoff(bmv, Jh, 2.f * (ph.mu * harden), ph.lambda * harden, tau);
                 } else {
                     float U[9], sig[3], V[9], scv[3];
                     svd3(Fh, U, sig, V);

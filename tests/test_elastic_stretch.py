@@ -34,10 +34,10 @@ int main()
         float F[9];
         for (int e = 0; e < 9; ++e)
             if (std::scanf("%f", &F[e]) != 1) return 3;
-        mpm::Sym3 v{};
+        mpm::Sym3 v{}, bmv{};
         float J = 0.f, tau[9] = {0};
-        const bool ok = mpm::elastic_stretch(F, lo2, hi2, &v, &J);
-        if (ok) mpm::elastic_kirchhoff(v, J, tmu, la, tau);
+        const bool ok = mpm::elastic_stretch(F, lo2, hi2, &v, &bmv, &J);
+        if (ok) mpm::elastic_kirchhoff(bmv, J, tmu, la, tau);
         std::printf("%d %.9g %.9g %.9g %.9g %.9g %.9g %.9g", ok ? 1 : 0, v.xx, v.yy, v.zz, v.xy, v.xz, v.yz, J);
         for (int e = 0; e < 9; ++e) std::printf(" %.9g", tau[e]);
         std::printf("\n");
