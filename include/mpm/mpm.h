@@ -102,7 +102,7 @@ int mpm_reset_grid(mpm_solver* s);
 
 /* Particle binning: key = cell id in block-major order (with fused 2 or 3:
  * super-block-major, super-block s of an axis holding cells 8s-7 .. 8s, then the
- * 4^3-cell quarter, then the cell; the periodic re-binning of mpm_step
+ * cell within the super-block, z slowest; the periodic re-binning of mpm_step
  * then sorts by the super-block number alone, keeping the previous order inside
  * each super-block), stable LSD radix sort, reorder of every particle array.
  * mpm_get_sort returns the keys of the last binning.  No reference counterpart

@@ -88,12 +88,13 @@ __device__ inline int fixed_cap(int cnt, float wmax)
 
 // Largest |m| and largest momentum bound of each item's particles in the stored
 // state (shaders/3d/p2g.glsl:44-59 stress), i.e. what a scatter of that state
-// would need, and their mean position (the reference of the tile's drift), with a
-// zero tile offset: written after every binning.
+// would need, their mean position (the reference of the tile's drift) and the
+// z-range of their stencil bases, with a zero tile offset: written after every
+// binning.
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork work, Phys ph)
 {
-    __shared__ float s_red[5][4];
+    __shared__ float s_red[7][4];
     const int n = P.n;
     const int nit = *work.n_items;
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
@@ -102,10 +103,14 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
         int sc[3];
         super_coords(item.x, work.nsx, work.nsy, sc);
         float bm = 0.f, bpm = 0.f, msum[3] = {0.f, 0.f, 0.f};
+        int zlo = 1 << 20, zhi = -(1 << 20);
         for (int q = threadIdx.x; q < cnt; q += 256) {
             const int p = item.y + q;
 #pragma unroll
             for (int d = 0; d < 3; ++d) msum[d] += P.x[d * n + p] * ph.inv_dx - (float)(sc[d] * SB_CELLS);
+            const int bz = Spline<ORDER>::base(P.x[2 * n + p] * ph.inv_dx);
+            zlo = min(zlo, bz);
+            zhi = max(zhi, bz);
             const float vol = P.vol[p];
             if (vol == 0.f) continue;                                // p2g.glsl:42
             const float pm = P.mass[p], Jp = P.Jp[p];
@@ -137,11 +142,14 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
         bpm = wave_max(bpm);
 #pragma unroll
         for (int d = 0; d < 3; ++d) msum[d] = wave_sum(msum[d]);
+        const float nzlo = wave_max(-(float)zlo), fzhi = wave_max((float)zhi);
         if ((threadIdx.x & 63) == 0) {
             s_red[0][threadIdx.x >> 6] = bm;
             s_red[1][threadIdx.x >> 6] = bpm;
 #pragma unroll
             for (int d = 0; d < 3; ++d) s_red[2 + d][threadIdx.x >> 6] = msum[d];
+            s_red[5][threadIdx.x >> 6] = nzlo;
+            s_red[6][threadIdx.x >> 6] = fzhi;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -152,7 +160,9 @@ __global__ void __launch_bounds__(256) k_item_bounds(DevParticles P, SuperWork w
 #pragma unroll
             for (int d = 0; d < 3; ++d) mean[d] = ((s_red[2 + d][0] + s_red[2 + d][1]) + (s_red[2 + d][2] + s_red[2 + d][3])) * inv;
             work.ref[it] = make_float4(mean[0], mean[1], mean[2], 0.f);
-            work.shift[it] = make_int4(0, 0, 0, 0);
+            const int lo = -(int)fmaxf(fmaxf(s_red[5][0], s_red[5][1]), fmaxf(s_red[5][2], s_red[5][3]));
+            const int hi = (int)fmaxf(fmaxf(s_red[6][0], s_red[6][1]), fmaxf(s_red[6][2], s_red[6][3]));
+            work.shift[it] = make_int4(0, 0, 0, pack_zrange(lo, hi));
         }
         __syncthreads();
     }
@@ -175,7 +185,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
     // velocity in rows 3..5 with Coulomb walls
     __shared__ float s_vel[G2P ? NVR : 1][G2P ? TN : 1];
     __shared__ int s_touch[NF * NF * NF];
-    __shared__ float s_red[5][4];
+    __shared__ float s_red[7][4];
 
     const int n = P.n;
     const int tid = threadIdx.x;
@@ -197,10 +207,18 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             o[a] = sc[a] * SB_CELLS - SB_CELLS - MG + sh[a];
             tb[a] = o[a] >> 2;                        // first grid block the tile overlaps
         }
+        // node planes (tile z) in use: the gather reads [g_lo, g_hi], the stencils
+        // of the positions the previous substep scattered from; the scatter adds to
+        // [r_lo, r_hi], a plane more each way for this substep's motion.  Particles
+        // beyond them take the per-node gather and the float scatter.
+        const int g_lo = max(zrange_lo(shift.w) - o[2], 0), g_hi = min(zrange_hi(shift.w) - o[2] + W - 1, T - 1);
+        const int r_lo = max(zrange_lo(shift.w) - o[2] - 1, 0), r_hi = min(zrange_hi(shift.w) - o[2] + W, T - 1);
 
-        // ---- stage the tile (all loads issued before the first LDS store)
+        // ---- stage the tile's planes in use (all loads issued before the first
+        // LDS store)
         int ltid = tid;
         asm volatile("" : "+v"(ltid));
+        const int r_beg = r_lo * (T * T), r_end = (r_hi + 1) * (T * T);
         {
             constexpr int NR = (TN + 255) / 256;
             float4 va[NR], vb[NR];
@@ -208,10 +226,11 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             for (int k = 0; k < NR; ++k) {
                 va[k] = make_float4(0.f, 0.f, 0.f, 0.f);
                 vb[k] = va[k];
-                const int e = ltid + 256 * k;
-                if (G2P && e < TN) {
-                    const int i = o[0] + e % T, j = o[1] + (e / T) % T, kk = o[2] + e / (T * T);
-                    if (in_grid<3>(ph, i, j, kk)) {
+                const int e = r_beg + ltid + 256 * k;
+                if (G2P && e < r_end) {
+                    const int pz = e / (T * T);
+                    const int i = o[0] + e % T, j = o[1] + (e / T) % T, kk = o[2] + pz;
+                    if (pz >= g_lo && pz <= g_hi && in_grid<3>(ph, i, j, kk)) {
                         const int addr = node_addr<3>(src, i, j, kk);
                         va[k] = src.a[addr];
                         if constexpr (COULOMB) vb[k] = src.vc[addr];
@@ -220,8 +239,8 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             }
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int e = ltid + 256 * k;
-                if (e >= TN) continue;
+                const int e = r_beg + ltid + 256 * k;
+                if (e >= r_end) continue;
                 s_acc[0][e] = 0ull;
                 s_acc[1][e] = 0ull;
                 if constexpr (G2P) {
@@ -250,6 +269,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         lds_barrier();
 
         float bound_m = 0.f, bound_p = 0.f, msum[3] = {0.f, 0.f, 0.f};
+        int zmn = 1 << 20, zmx = -(1 << 20);
         const int npass = (cnt + 63) >> 6;
         for (int pass = ((tid >> 6) + it) & 3; pass < npass; pass += 4) {
             const int q = pass * 64 + (tid & 63);
@@ -276,6 +296,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                     rb[a] = st.base[a] - o[a];
                     tile_ok &= rb[a] >= 0 && rb[a] + W <= T;
                 }
+                tile_ok &= rb[2] >= g_lo && rb[2] + W - 1 <= g_hi;
                 inside = tile_ok;
                 {
                     const int lim3[3] = {ph.W, ph.H, ph.D};
@@ -477,6 +498,11 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 
 #pragma unroll
             for (int d = 0; d < 3; ++d) msum[d] += x[d] * ph.inv_dx - (float)(sc[d] * SB_CELLS);
+            {
+                const int bz = Spline<ORDER>::base(x[2] * ph.inv_dx);
+                zmn = min(zmn, bz);
+                zmx = max(zmx, bz);
+            }
             if (vol == 0.f) continue;                                 // p2g.glsl:42
             // ---- scatter (p2g.glsl): into the tile when the stencil lies in it and
             // the contributions fit the item's scale, else global float atomics
@@ -501,6 +527,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                     tile_ok &= rb[d] >= 0 && rb[d] + W <= T;
                     inside &= base[d] >= 0 && base[d] + W - 1 <= lim3[d];
                 }
+                tile_ok &= rb[2] >= r_lo && rb[2] + W - 1 <= r_hi;
             }
             float wa[3][W], xa[3][W];
 #pragma unroll
@@ -609,13 +636,21 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         bound_p = wave_max(bound_p);
 #pragma unroll
         for (int d = 0; d < 3; ++d) msum[d] = wave_sum(msum[d]);
+        const float nzmn = wave_max(-(float)zmn), fzmx = wave_max((float)zmx);
         if ((tid & 63) == 0) {
             s_red[0][tid >> 6] = bound_m;
             s_red[1][tid >> 6] = bound_p;
 #pragma unroll
             for (int d = 0; d < 3; ++d) s_red[2 + d][tid >> 6] = msum[d];
+            s_red[5][tid >> 6] = nzmn;
+            s_red[6][tid >> 6] = fzmx;
         }
         lds_barrier();
+        // z-range of the stencil bases this substep scattered from (every particle's
+        // new position): the next substep's gather planes, and with [r_lo, r_hi] the
+        // planes to flush
+        const int zlo = -(int)fmaxf(fmaxf(s_red[5][0], s_red[5][1]), fmaxf(s_red[5][2], s_red[5][3]));
+        const int zhi = (int)fmaxf(fmaxf(s_red[6][0], s_red[6][1]), fmaxf(s_red[6][2], s_red[6][3]));
         if (tid == 0) {
             work.bound[it] = make_float2(fmaxf(fmaxf(s_red[0][0], s_red[0][1]), fmaxf(s_red[0][2], s_red[0][3])),
                                          fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3])));
@@ -628,7 +663,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 #pragma unroll
             for (int d = 0; d < 3; ++d)
                 ns[d] = drift_shift(((s_red[2 + d][0] + s_red[2 + d][1]) + (s_red[2 + d][2] + s_red[2 + d][3])) * inv - rf[d]);
-            work.shift[it] = make_int4(ns[0], ns[1], ns[2], 0);
+            work.shift[it] = make_int4(ns[0], ns[1], ns[2], pack_zrange(zlo, zhi));
         }
 
         // ---- flush: fixed point back to FP32, one global atomic per touched node
@@ -643,11 +678,12 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
         // four passes' LDS words read together, so that their latencies overlap
         constexpr int FU = 4;
         const int c = tid & 3;
-        for (int pe0 = tid; pe0 < TN * 4; pe0 += 256 * FU) {
+        const int f_beg = max(zlo - o[2], r_lo) * (T * T * 4), f_end = (min(zhi - o[2] + W - 1, r_hi) + 1) * (T * T * 4);
+        for (int pe0 = f_beg + tid; pe0 < f_end; pe0 += 256 * FU) {
             int acc[FU], mass[FU];
 #pragma unroll
             for (int u = 0; u < FU; ++u) {
-                const int e = min(pe0 + 256 * u, TN * 4 - 1) >> 2;
+                const int e = min(pe0 + 256 * u, f_end - 1) >> 2;
                 const unsigned long long sum = s_acc[c >> 1][e];
                 acc[u] = (c & 1) ? unpack_hi(sum) : unpack_lo(sum);
                 mass[u] = unpack_lo(s_acc[0][e]);
@@ -659,7 +695,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 // and momentum round independently, and a momentum unit over a
                 // float-path mass far below the mass unit would give the node any
                 // velocity
-                if (pe >= TN * 4 || acc[u] == 0 || mass[u] == 0) continue;
+                if (pe >= f_end || acc[u] == 0 || mass[u] == 0) continue;
                 const int e = pe >> 2;
                 const int i = o[0] + e % T, j = o[1] + (e / T) % T, k = o[2] + e / (T * T);
                 if (!tile_in && !in_grid<3>(ph, i, j, k)) continue;
@@ -683,7 +719,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 
 // Super-block binning keys: the cell containing the particle, clamped into the
 // grid, numbered super-block-major: key = ((sz * nsy + sy) * nsx + sx) * 512 +
-// (block within the super-block) * 64 + (cell within the block, x fastest), or with
+// (cell within the super-block, z slowest and x fastest), or with
 // cells = false the super-block number alone.
 __global__ void __launch_bounds__(256) k_cell_keys_super(const float* x, int n, float inv_dx, int W, int H, int D,
                                                          int nsx, int nsy, bool cells, uint32_t* keys)

@@ -21,8 +21,10 @@ struct SuperWork {
     float2*     bound;   // per item: largest |m| and largest momentum bound of its particles
     float4*     ref;     // per item: its particles' mean position at binning, in cells from
                          // the super-block's first cell
-    int4*       shift;   // per item: the tile's offset in cells, the rounded mean drift of
-                         // its particles since binning
+    int4*       shift;   // per item: in x, y, z the tile's offset in cells, the rounded mean
+                         // drift of its particles since binning; in w the z-range of their
+                         // stencil bases (pack_zrange), from the positions the last substep
+                         // scattered from, i.e. the ones the next gather reads at
     int         nsx, nsy;   // super-blocks along x and y
 };
 
@@ -42,6 +44,19 @@ __host__ __device__ inline int drift_shift(float d)
     return fabsf(d) < 64.f ? (int)rintf(d) : 0;
 }
 
+// Lowest and highest z of an item's stencil bases (grid nodes) in one int, 16 bits
+// each, clamped to the 16-bit range.  A tile stages and flushes only the node planes
+// that range needs; an empty or stale range costs speed, never results (a particle
+// outside the staged planes gathers from HBM and scatters with global atomics).
+__host__ __device__ inline int pack_zrange(int lo, int hi)
+{
+    lo = lo < -32768 ? -32768 : (lo > 32767 ? 32767 : lo);
+    hi = hi < -32768 ? -32768 : (hi > 32767 ? 32767 : hi);
+    return (int)(((unsigned)hi << 16) | ((unsigned)lo & 0xffffu));
+}
+__host__ __device__ inline int zrange_lo(int packed) { return (int)(short)(packed & 0xffff); }
+__host__ __device__ inline int zrange_hi(int packed) { return packed >> 16; }
+
 // Super-block s along an axis holds cells [8 s - 7, 8 s]: the quadratic stencils of
 // its particles span nodes [8 s - 8, 8 s + 2], so its tile starts on a grid-block
 // boundary and each tile row covers 3 (not 4) blocks' 64-byte node runs.
@@ -52,18 +67,17 @@ __host__ __device__ constexpr int super_of_cell(int c) { return (c + SB_CELL_SHI
 __host__ __device__ constexpr int super_count(int cells) { return super_of_cell(cells) + 1; }
 
 // Binning key of cell (cx, cy, cz) (inside the grid): super-block-major, then the
-// 4^3-cell quarter within the super-block, then the cell within that, x fastest; with
-// cells = false the super-block alone (key >> SB_KEY_SHIFT of the full key), so that a
-// stable sort keeps the previous order inside each super-block.
+// cell within the super-block in z-plane order (z slowest, x fastest), so that an
+// item holding part of a super-block spans few of its cell planes; with cells = false
+// the super-block alone (key >> SB_KEY_SHIFT of the full key), so that a stable sort
+// keeps the previous order inside each super-block.
 __host__ __device__ inline uint32_t super_key(int cx, int cy, int cz, int nsx, int nsy, bool cells = true)
 {
     const uint32_t sb = ((uint32_t)super_of_cell(cz) * (uint32_t)nsy + (uint32_t)super_of_cell(cy)) * (uint32_t)nsx +
                         (uint32_t)super_of_cell(cx);
     if (!cells) return sb;
     const int lx = (cx + SB_CELL_SHIFT) & 7, ly = (cy + SB_CELL_SHIFT) & 7, lz = (cz + SB_CELL_SHIFT) & 7;
-    const uint32_t lb = ((lz >> 2) << 2) | ((ly >> 2) << 1) | (lx >> 2);
-    const uint32_t cell = ((lz & 3) << 4) | ((ly & 3) << 2) | (lx & 3);
-    return (sb << SB_KEY_SHIFT) | (lb << 6) | cell;
+    return (sb << SB_KEY_SHIFT) | (uint32_t)((lz << 6) | (ly << 3) | lx);
 }
 
 hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx, int nsy, bool cells,

@@ -94,8 +94,8 @@ def super_keys(params: Params, x: np.ndarray) -> np.ndarray:
     """CPU restatement of the 3D super-block binning key (mpm_options.fused 2 or 3):
     the cell that holds the particle (clamped into the grid), numbered
     super-block-major (super-block s along an axis holds cells 8s-7 .. 8s, so that
-    its particles' stencils start on a grid-block boundary), then by the 4^3-cell
-    quarter within the super-block, then by the cell within that, x fastest."""
+    its particles' stencils start on a grid-block boundary), then by the cell within
+    the super-block in z-plane order (z slowest, x fastest)."""
     inv = np.float32(1.0) / np.float32(params.dx)
     lim = [params.WIDTH, params.HEIGHT, params.DEPTH]
     c = [np.clip(np.floor(np.asarray(x[a], np.float32) * inv).astype(np.int64), 0, lim[a]) for a in range(3)]
@@ -103,9 +103,8 @@ def super_keys(params: Params, x: np.ndarray) -> np.ndarray:
     s = [(ca + 7) >> 3 for ca in c]
     loc = [(ca + 7) & 7 for ca in c]
     sb = (s[2] * nsy + s[1]) * nsx + s[0]
-    lb = ((loc[2] >> 2) << 2) | ((loc[1] >> 2) << 1) | (loc[0] >> 2)
-    cell = ((loc[2] & 3) << 4) | ((loc[1] & 3) << 2) | (loc[0] & 3)
-    return ((sb << 9) | (lb << 6) | cell).astype(np.uint32)
+    cell = (loc[2] << 6) | (loc[1] << 3) | loc[0]
+    return ((sb << 9) | cell).astype(np.uint32)
 
 
 def super_block_ids(params: Params, x: np.ndarray) -> np.ndarray:
