@@ -138,6 +138,15 @@ template <> struct Spline<3> {
         if (a < 2.0f) return (-0.5f * a * x) + (2.0f * x) - ((2.0f * x) / a);
         return 0.0f;
     }
+    // The stencil's weights w[s] = N(d[s]) and distances d[s] = xg - (base + s).
+    __device__ static inline void weights(float xg, int base, float* w, float* d)
+    {
+#pragma unroll
+        for (int s = 0; s < W; ++s) {
+            d[s] = xg - (float)(base + s);
+            w[s] = N(d[s]);
+        }
+    }
 };
 
 template <> struct Spline<2> {
@@ -157,6 +166,25 @@ template <> struct Spline<2> {
         if (a < 1.5f) return (x > 0.0f ? -1.0f : 1.0f) * (1.5f - a);
         return 0.0f;
     }
+    // The stencil's weights w[s] = N(d[s]) and distances d[s] = xg - (base + s) for
+    // base = base(xg): d[0] lies in [0.5, 1.5), so each node's piece of N is known
+    // and no branch is needed (N is C1 where the pieces meet, so a base that the
+    // rounding of xg - 0.5 moves by one changes a weight only in its last bits).  A
+    // non-finite xg, or one so large that its base was clamped, gets N's zeros
+    // beyond the support, as N itself gives.
+    __device__ static inline void weights(float xg, int base, float* w, float* d)
+    {
+        const float d0 = xg - (float)base;
+        const float d1 = d0 - 1.0f;
+        d[0] = d0;
+        d[1] = d1;
+        d[2] = d0 - 2.0f;
+        const float t0 = 0.5f - d1, t2 = 0.5f + d1;
+        const bool ok = fabsf(d1) <= 1.0f;
+        w[0] = ok ? 0.5f * t0 * t0 : 0.0f;
+        w[1] = ok ? 0.75f - d1 * d1 : 0.0f;
+        w[2] = ok ? 0.5f * t2 * t2 : 0.0f;
+    }
 };
 
 // Per-particle stencil: base node, per-axis weights and grid-unit distances
@@ -172,11 +200,7 @@ struct Stencil {
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
             base[a] = Spline<ORDER>::base(xg[a]);
-#pragma unroll
-            for (int s = 0; s < W; ++s) {
-                d[a][s] = xg[a] - (float)(base[a] + s);
-                w[a][s] = Spline<ORDER>::N(d[a][s]);
-            }
+            Spline<ORDER>::weights(xg[a], base[a], w[a], d[a]);
         }
     }
 };

@@ -610,12 +610,10 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
             for (int d = 0; d < DIM; ++d) {
                 base[d] = Spline<ORDER>::base(in[r].xg[d]);
                 rb[d] = base[d] - o[d];
+                float dd[W];
+                Spline<ORDER>::weights(in[r].xg[d], base[d], wa[d], dd);
 #pragma unroll
-                for (int s2 = 0; s2 < W; ++s2) {
-                    const float dd = in[r].xg[d] - (float)(base[d] + s2);
-                    wa[d][s2] = Spline<ORDER>::N(dd);
-                    xa[d][s2] = -dd * ph.dx;
-                }
+                for (int s2 = 0; s2 < W; ++s2) xa[d][s2] = -dd[s2] * ph.dx;
             }
             const bool via_tile = in[r].mode == 1;
             bool inside = via_tile;

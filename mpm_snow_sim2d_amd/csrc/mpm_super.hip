@@ -531,13 +531,12 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
             }
             float wa[3][W], xa[3][W];
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
+                float dd[W];
+                Spline<ORDER>::weights(xg[d], base[d], wa[d], dd);
 #pragma unroll
-                for (int s2 = 0; s2 < W; ++s2) {
-                    const float dd = xg[d] - (float)(base[d] + s2);
-                    wa[d][s2] = Spline<ORDER>::N(dd);
-                    xa[d][s2] = -dd * ph.dx;
-                }
+                for (int s2 = 0; s2 < W; ++s2) xa[d][s2] = -dd[s2] * ph.dx;
+            }
             if (tile_ok && inside) {
                 // whole stencil in the tile and the grid: scaled inputs, one rounding
                 // per component, two packed 64-bit LDS adds per node.  Each lane walks
