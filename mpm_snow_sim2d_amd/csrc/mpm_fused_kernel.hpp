@@ -59,6 +59,56 @@ constexpr unsigned FIXED_BIAS = 0x4B400000u;
 
 __device__ inline int fixed_bits(float a, float b) { return __float_as_int(fmaf(a, b, 12582912.0f)); }
 
+// v / W for 0 <= v < 128 without a division: for W = 3 a multiply and shift (86/256
+// exceeds 1/3 by less than 1/(3 * 128)), for W = 4 a shift.
+template <int W>
+__device__ inline unsigned small_div(unsigned v)
+{
+    static_assert(W == 3 || W == 4, "stencil width");
+    if constexpr (W == 4) return v >> 2;
+    else return __umul24(v, 86u) >> 8;
+}
+
+// One axis of a lane's rotated stencil walk.  At step s the lane takes the node at
+// offset (s + rot) mod W from its base, rot = (phase - rb) mod W, whose residue mod W
+// is then phase + s whatever the base.  wa, xa: the weights and distance terms in
+// node order; wr, xr: the same in walk order; er: the walk's node offsets along the
+// axis (tile base rb >= 0 included) times the axis stride.  Every output is one
+// select per nonzero rotation from the inputs, on W - 1 compares of rot.
+template <int W>
+__device__ inline void walk_axis(int phase, int rb, int strd, const float* wa, const float* xa, float* wr,
+                                 float* xr, int* er)
+{
+    // phase + 96 - rb lies in (0, 128) for rb in [0, 96), and 96 is a multiple of W
+    const unsigned v = (unsigned)(phase + 96 - rb);
+    const int rot = (int)(v - small_div<W>(v) * W);
+    const int rbs = __umul24(rb, strd);
+    // the inputs as opaque values, so that the selects stay selects between
+    // registers (not a load from a private array at a computed index)
+    float a[W], b[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        a[s] = wa[s];
+        b[s] = xa[s];
+        asm volatile("" : "+v"(a[s]), "+v"(b[s]));
+    }
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+        float wv = a[s], xv = b[s];
+        int ov = s * strd;
+#pragma unroll
+        for (int k = 1; k < W; ++k)
+            if (rot == k) {
+                wv = a[(s + k) % W];
+                xv = b[(s + k) % W];
+                ov = ((s + k) % W) * strd;
+            }
+        wr[s] = wv;
+        xr[s] = xv;
+        er[s] = rbs + ov;
+    }
+}
+
 __device__ inline unsigned long long pack_fixed(int lo_bits, int hi_bits)
 {
     constexpr unsigned long long B = ((unsigned long long)FIXED_BIAS << 32) | FIXED_BIAS;
@@ -159,6 +209,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
 {
     using TS = TileShape<DIM>;
     constexpr int BS = TS::BS, MG = TS::MG, T = TS::T, TN = TS::TN;
+    static_assert(T <= 96, "walk_axis takes tile bases below 96");
     constexpr int W = Spline<ORDER>::W;
     constexpr int TV = G2P ? TN : 1;
     constexpr int RMAX = 2;                      // particles per thread per item
@@ -648,28 +699,16 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                     // the lane's rotation, from an opaque copy of its index: derived per
                     // particle (a few integer operations) instead of being hoisted out
                     // of the item loop as compare masks kept live (or spilled) across G2P
-                    int lr = tid & 63;
+                    unsigned lr = tid & 63;
                     asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
                         // the lane's phase (its index's base-W digits) fixes
                         // (base + rot) mod W, so that lanes of different phases never
                         // add to one word at the same step (as in k_super)
-                        const int phase = lr % W;
-                        lr /= W;
-                        int rot = phase - rb[d] % W;
-                        rot = rot < 0 ? rot + W : rot;
-#pragma unroll
-                        for (int s2 = 0; s2 < W; ++s2) {
-                            const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
-                            float wv = wa[d][0], xv = xa[d][0];
-#pragma unroll
-                            for (int t2 = 1; t2 < W; ++t2)
-                                if (idx == t2) { wv = wa[d][t2]; xv = xa[d][t2]; }
-                            wr[d][s2] = wv;
-                            xr[d][s2] = xv;
-                            er[d][s2] = (rb[d] + idx) * strd[d];
-                        }
+                        const unsigned q = small_div<W>(lr);
+                        walk_axis<W>((int)(lr - q * W), rb[d], strd[d], wa[d], xa[d], wr[d], xr[d], er[d]);
+                        lr = q;
                     }
                 }
 #pragma unroll

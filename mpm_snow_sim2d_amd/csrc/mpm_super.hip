@@ -174,6 +174,7 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 {
     constexpr int T = SB_CELLS + 3 + 2 * MG;          // tile edge (nodes)
     constexpr int TN = T * T * T;
+    static_assert(T <= 96, "walk_axis takes tile bases below 96");
     // grid blocks a tile overlaps per axis: ((o & 3) + T - 1) / 4 + 1 <= 4 for T <= 13
     constexpr int NF = SB_BLOCKS + 2;
     static_assert((3 + T - 1) / 4 + 1 <= NF, "tile overlaps more grid blocks than s_touch holds");
@@ -559,25 +560,13 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 int er[3][W];
                 {
                     const int strd[3] = {1, T, T * T};
-                    int lr = tid & 63;
+                    unsigned lr = tid & 63;
                     asm volatile("" : "+v"(lr));
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        const int phase = lr % W;
-                        lr /= W;
-                        int rot = phase - rb[d] % W;       // rb[d] >= 0 in the tile
-                        rot = rot < 0 ? rot + W : rot;
-#pragma unroll
-                        for (int s2 = 0; s2 < W; ++s2) {
-                            const int idx = s2 + rot >= W ? s2 + rot - W : s2 + rot;
-                            float wv = wa[d][0], xv = xa[d][0];
-#pragma unroll
-                            for (int t2 = 1; t2 < W; ++t2)
-                                if (idx == t2) { wv = wa[d][t2]; xv = xa[d][t2]; }
-                            wr[d][s2] = wv;
-                            xr[d][s2] = xv;
-                            er[d][s2] = (rb[d] + idx) * strd[d];
-                        }
+                        const unsigned q = small_div<W>(lr);      // rb[d] >= 0 in the tile
+                        walk_axis<W>((int)(lr - q * W), rb[d], strd[d], wa[d], xa[d], wr[d], xr[d], er[d]);
+                        lr = q;
                     }
                 }
 #pragma unroll
