@@ -237,20 +237,24 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
     __shared__ int s_touch[TS::NB];
     __shared__ float s_red[2][4];
 
-    // friction (vF) and collision (vC) velocity of tile node e from the staged tile
-    auto load_node_velocity = [&](int e, float* vF, float* vC) {
+    // friction (vF) and collision (vC) velocity of tile node e0 + off from the staged
+    // tile (off a constant where the caller has one: it then folds into the reads'
+    // offsets)
+    auto load_node_velocity = [&](int e0, int off, float* vF, float* vC) {
         if constexpr (VSOA) {
+            const float* sv = &s_vel[0][0] + e0;
             vF[2] = vC[2] = 0.f;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                vF[d] = s_vel[d][e];
-                vC[d] = COULOMB ? s_vel[(DIM + d) % NVR][e] : vF[d];
+                vF[d] = sv[d * TV + off];
+                vC[d] = COULOMB ? sv[((DIM + d) % NVR) * TV + off] : vF[d];
             }
         } else {
-            const float4 f4 = s_vel4[0][e];
+            const float4* s4 = &s_vel4[0][0] + e0;
+            const float4 f4 = s4[off];
             vF[0] = f4.x; vF[1] = f4.y; vF[2] = f4.z;
             if constexpr (COULOMB) {
-                const float4 c4 = s_vel4[COULOMB ? 1 : 0][e];
+                const float4 c4 = s4[(COULOMB ? TV : 0) + off];
                 vC[0] = c4.x; vC[1] = c4.y; vC[2] = c4.z;
             } else {
                 vC[0] = vF[0]; vC[1] = vF[1]; vC[2] = vF[2];
@@ -422,7 +426,7 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                                 if (decltype(fast)::value || tile_ok) {
                                     int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
                                                      : (rb[1] + a1) * T + (rb[0] + a0);
-                                    load_node_velocity(e, vF, vC);
+                                    load_node_velocity(e, 0, vF, vC);
                                 } else {
                                     int addr = node_addr<DIM>(src, idx[0], idx[1], idx[2]);
                                     float4 a4 = src.a[addr];
@@ -460,6 +464,8 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                     float nc[DIM];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) nc[d] = 0.f;
+                    // the stencil's first tile node; the others at constant offsets
+                    const int e0 = DIM == 3 ? (rb[2] * T + rb[1]) * T + rb[0] : rb[1] * T + rb[0];
 #pragma unroll
                     for (int a0 = 0; a0 < W; ++a0) {
                         // sums over a1 (and a2): S v, D1 = sum wd1 v, D2 = sum wd2 v, Cs vC
@@ -473,10 +479,9 @@ __global__ void __launch_bounds__(256, 3) k_tiled(DevParticles P, DevGrid src_al
                             for (int d = 0; d < DIM; ++d) { S2[d] = 0.f; E2[d] = 0.f; C2[d] = 0.f; }
 #pragma unroll
                             for (int a2 = 0; a2 < (DIM == 3 ? W : 1); ++a2) {
-                                const int e = DIM == 3 ? ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0)
-                                                       : (rb[1] + a1) * T + (rb[0] + a0);
+                                const int off = DIM == 3 ? (a2 * T + a1) * T + a0 : a1 * T + a0;
                                 float vF[3], vC[3];
-                                load_node_velocity(e, vF, vC);
+                                load_node_velocity(e0, off, vF, vC);
                                 if constexpr (DIM == 3) {
                                     const float w2 = st.w[DIM - 1][a2], wd2 = wd[DIM - 1][a2];
 #pragma unroll

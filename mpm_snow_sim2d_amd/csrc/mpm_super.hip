@@ -316,6 +316,9 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 #pragma unroll
                         for (int s2 = 0; s2 < W; ++s2) wd[a][s2] = -st.w[a][s2] * (st.d[a][s2] * ph.dx);
                     float nc[3] = {0.f, 0.f, 0.f};
+                    // the stencil's first node in the tile; the others sit at constant
+                    // offsets from it (no per-node index arithmetic)
+                    const float* sv = &s_vel[0][0] + ((rb[2] * T + rb[1]) * T + rb[0]);
 #pragma unroll
                     for (int a0 = 0; a0 < W; ++a0) {
                         float S1[3] = {0.f, 0.f, 0.f}, D1[3] = {0.f, 0.f, 0.f}, D2[3] = {0.f, 0.f, 0.f},
@@ -325,14 +328,14 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                             float S2[3] = {0.f, 0.f, 0.f}, E2[3] = {0.f, 0.f, 0.f}, C2[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                             for (int a2 = 0; a2 < W; ++a2) {
-                                const int e = ((rb[2] + a2) * T + (rb[1] + a1)) * T + (rb[0] + a0);
+                                const int off = (a2 * T + a1) * T + a0;
                                 const float w2 = st.w[2][a2], wd2 = wd[2][a2];
 #pragma unroll
                                 for (int d = 0; d < 3; ++d) {
-                                    const float vF = s_vel[d][e];
+                                    const float vF = sv[d * TN + off];
                                     S2[d] += w2 * vF;
                                     E2[d] += wd2 * vF;
-                                    if constexpr (COULOMB) C2[d] += w2 * s_vel[(3 + d) % NVR][e];
+                                    if constexpr (COULOMB) C2[d] += w2 * sv[((3 + d) % NVR) * TN + off];
                                 }
                             }
                             const float w1 = st.w[1][a1], wd1 = wd[1][a1];
