@@ -309,12 +309,17 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                 for (int e = 0; e < 9; ++e) nB[e] = 0.f;
                 if (inside) {
                     // the node weight is a product of per-axis weights, so the sums
-                    // over the W^3 nodes factor axis by axis (as in k_tiled)
-                    float wd[3][W];
+                    // over the W^3 nodes factor axis by axis (as in k_tiled).  Along z,
+                    // sum_s w_s (d_0 - s) v_s = d_0 sum_s w_s v_s - sum_s s w_s v_s: the
+                    // second sum (one term fewer) is carried instead of the weighted
+                    // distances, and the first is the velocity sum itself
+                    float wd[2][W], kw[W];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a)
+                    for (int a = 0; a < 2; ++a)
 #pragma unroll
                         for (int s2 = 0; s2 < W; ++s2) wd[a][s2] = -st.w[a][s2] * (st.d[a][s2] * ph.dx);
+#pragma unroll
+                    for (int s2 = 0; s2 < W; ++s2) kw[s2] = (float)s2 * st.w[2][s2];
                     float nc[3] = {0.f, 0.f, 0.f};
                     // the stencil's first node in the tile; the others sit at constant
                     // offsets from it (no per-node index arithmetic)
@@ -329,12 +334,12 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
 #pragma unroll
                             for (int a2 = 0; a2 < W; ++a2) {
                                 const int off = (a2 * T + a1) * T + a0;
-                                const float w2 = st.w[2][a2], wd2 = wd[2][a2];
+                                const float w2 = st.w[2][a2];
 #pragma unroll
                                 for (int d = 0; d < 3; ++d) {
                                     const float vF = sv[d * TN + off];
                                     S2[d] += w2 * vF;
-                                    E2[d] += wd2 * vF;
+                                    if (a2 > 0) E2[d] += kw[a2] * vF;
                                     if constexpr (COULOMB) C2[d] += w2 * sv[((3 + d) % NVR) * TN + off];
                                 }
                             }
@@ -357,6 +362,9 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
                             if constexpr (COULOMB) nc[d] += w0 * C1[d];
                         }
                     }
+                    // z: -dx (d_0 sum w v - sum s w v)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) nB[d * 3 + 2] = (nB[d * 3 + 2] - st.d[2][0] * nv[d]) * ph.dx;
                     float sw[3], mi[3];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
