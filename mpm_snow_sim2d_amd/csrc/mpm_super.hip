@@ -192,14 +192,9 @@ __global__ void __launch_bounds__(256, 3) k_super(DevParticles P, DevGrid src, D
     const int tid = threadIdx.x;
     const int nit = *work.n_items;
 
-    // Workgroups go to the 8 XCDs in turn (workgroup b to XCD b mod 8), each XCD with
-    // its own L2, while consecutive items share grid nodes (a super-block's items,
-    // their tile halos): each XCD takes runs of SB_XCD_RUN consecutive items, the runs
-    // dealt to the XCDs in turn (so that dense and sparse regions still spread over
-    // all of them), in rounds of gridDim.x items (a multiple of 8 SB_XCD_RUN).
+    // XCD-aware: each XCD takes runs of consecutive items (xcd_item_slot)
     const int G = gridDim.x;
-    const int k8 = (int)(blockIdx.x >> 3);
-    const int slot = ((k8 / SB_XCD_RUN) * 8 + (int)(blockIdx.x & 7)) * SB_XCD_RUN + k8 % SB_XCD_RUN;
+    const int slot = xcd_item_slot();
     for (int r0 = 0; r0 < nit; r0 += G) {
         const int it = r0 + slot;
         if (it >= nit) continue;
@@ -753,7 +748,7 @@ hipError_t launch_cell_keys_super(const float* x, int n, const Phys& ph, int nsx
 
 static int super_grid(const SuperWork& w)
 {
-    constexpr int R = 8 * SB_XCD_RUN;        // k_super maps workgroups in rounds of R
+    constexpr int R = 8 * XCD_RUN;           // k_super maps items in rounds of R (xcd_item_slot)
     int g = w.max_items < 4096 ? w.max_items : 4096;
     g = (g + R - 1) / R * R;
     return g > 0 ? g : R;

@@ -13,7 +13,6 @@ constexpr int SB_BLOCKS = 2;                 // grid blocks per axis of a super-
 constexpr int SB_CELLS = 4 * SB_BLOCKS;      // cells per axis of a super-block
 constexpr int SB_KEY_SHIFT = 9;              // key bits below the super-block id: 8 blocks x 64 cells
 constexpr int SB_MAX_PARTICLES = 4096;       // particles per work item at most
-constexpr int SB_XCD_RUN = 4;                // consecutive items one XCD takes in turn
 
 struct SuperWork {
     const int4* items;   // (super-block id, first particle, count, 0) per work item
